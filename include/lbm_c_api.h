@@ -1,0 +1,156 @@
+/*
+ * lbm_c_api.h - C ABI of the MI355X (gfx950) lattice-Boltzmann kernels.
+ *
+ * One header for the whole library (libmaddening_lbm.so). Plain pointers and
+ * sizes only: every pointer argument named f_*, mask, force_field, tau_field,
+ * rho, u, p, work, *_bar is a DEVICE pointer; `stream` is a hipStream_t passed
+ * as void*. No entry point allocates device memory: the caller passes every
+ * buffer, including the workspace sized by lbm_workspace_size(). All work is
+ * enqueued on `stream` and no entry point synchronises the host.
+ *
+ * Population layout ("SoA"): f[q][i0][i1][i2] for D3Q19 and f[q][i0][i1] for
+ * D2Q9, i.e. the reference's (*grid_shape, Q) array transposed to
+ * (Q, *grid_shape), last grid axis contiguous. Direction order, weights and
+ * opposite map are those of the reference's d3q19()/d2q9()
+ * (src/maddening/nodes/lbm.py:90-173).
+ *
+ * Storage types: LBM_F32/LBM_F64 hold f itself; LBM_F16_DEV/LBM_BF16_DEV
+ * hold the deviation f_q - w_q. Collision arithmetic is FP32 for F32/F16/BF16
+ * storage and FP64 for F64 storage.
+ *
+ * Auxiliary per-cell arrays use the reference's layouts (last grid axis
+ * contiguous): mask uint8 (*G) (0 fluid, 1 wall, 2 moving wall),
+ * force_field float (*G, D), tau_field float (*G), rho float (*G),
+ * u float (*G, D), p float (*G).
+ *
+ * Return value of every int function: 0 on success, a negative lbm_status on
+ * failure; lbm_last_error() then gives a message (thread-local).
+ */
+#ifndef MADDENING_LBM_C_API_H
+#define MADDENING_LBM_C_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_API_VERSION 1
+
+typedef enum lbm_status {
+  LBM_OK = 0,
+  LBM_ERR_INVALID = -1,     /* bad argument (null pointer, bad size, tau <= 0.5 ...) */
+  LBM_ERR_UNSUPPORTED = -2, /* combination not implemented */
+  LBM_ERR_HIP = -3          /* a HIP runtime call failed */
+} lbm_status;
+
+typedef enum lbm_storage {
+  LBM_F32 = 0,      /* float f */
+  LBM_F64 = 1,      /* double f */
+  LBM_F16_DEV = 2,  /* IEEE half, f - w */
+  LBM_BF16_DEV = 3  /* bfloat16, f - w */
+} lbm_storage;
+
+typedef enum lbm_bb_mode {
+  LBM_BB_NONE = 0,      /* fully periodic, mask ignored */
+  LBM_BB_REFERENCE = 1, /* the reference's node swap: f_q(x) <- f_stream_opp(q)(x) at wall cells */
+  LBM_BB_HALFWAY = 2    /* link-wise halfway bounce-back, moving wall for mask value 2 */
+} lbm_bb_mode;
+
+typedef enum lbm_scheme {
+  LBM_SCHEME_AUTO = 0,
+  LBM_SCHEME_PULL = 1,  /* stored state between kernels is post-collision */
+  LBM_SCHEME_PUSH = 2   /* stored state between kernels is pre-collision */
+} lbm_scheme;
+
+/* Lattice description shared by every entry point. */
+typedef struct lbm_desc {
+  int32_t q;            /* 9 (D2Q9) or 19 (D3Q19) */
+  int32_t storage;      /* lbm_storage */
+  int32_t bb_mode;      /* lbm_bb_mode */
+  int32_t scheme;       /* lbm_scheme; AUTO picks the measured-fastest */
+  int64_t dims[3];      /* grid shape in the reference's axis order; dims[2] ignored for D2Q9 */
+  double tau;           /* BGK relaxation time used where tau_field is NULL; must be > 0.5 */
+  double force[3];      /* uniform body force used where force_field is NULL */
+  double lid_velocity[3]; /* velocity of mask==2 cells in LBM_BB_HALFWAY */
+  int32_t block_x;      /* threads per block along the contiguous axis; 0 = auto */
+  int32_t flags;        /* LBM_FLAG_* */
+} lbm_desc;
+
+/* flags */
+#define LBM_FLAG_NO_XCD_REMAP 1  /* keep hardware workgroup order (default remaps rows per XCD) */
+
+/* ---- queries ----------------------------------------------------------- */
+
+/* Library ABI version (LBM_API_VERSION). */
+int lbm_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* lbm_last_error(void);
+
+/* Bytes of device workspace lbm_step() needs for `nsteps` steps (0 if none). */
+size_t lbm_workspace_size(const lbm_desc* d, int nsteps);
+
+/* Bytes of one population state (Q * cells * storage size). */
+size_t lbm_state_bytes(const lbm_desc* d);
+
+/* ---- forward ----------------------------------------------------------- */
+
+/*
+ * Advance `nsteps` (>= 1) steps of LBMNode.update
+ * (src/maddening/nodes/lbm.py:899-986): Guo-forced BGK collision on every
+ * cell, periodic streaming (lbm.py:242-267), bounce-back per d->bb_mode
+ * (reference mode: lbm.py:944-948), then the output moments of the final
+ * state (lbm.py:970-976): rho, u with the F/2 correction and u = 0 at mask
+ * cells, p = rho/3. rho/u/p may each be NULL to skip them.
+ * f_in and f_out are SoA states in d->storage and must not overlap.
+ * mask (nullable) selects wall cells: it plays the role of the reference's
+ * wall_mask_update / state["wall_mask"] (lbm.py:914-919).
+ * force_field (nullable) replaces d->force per cell (body_force of shape
+ * (*G, D), lbm.py:922-927); tau_field (nullable) replaces d->tau per cell.
+ * work: lbm_workspace_size(d, nsteps) bytes (may be NULL when that is 0).
+ */
+int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out,
+             void* work, const uint8_t* mask, const float* force_field,
+             const float* tau_field, float* rho, float* u, float* p,
+             int nsteps);
+
+/*
+ * Output moments of a state without stepping: _compute_macroscopic with the
+ * Guo F/2 half-correction (lbm.py:209-239), u zeroed at mask cells and
+ * p = rho * cs2 (lbm.py:971-976).
+ */
+int lbm_moments(void* stream, const lbm_desc* d, const void* f,
+                const uint8_t* mask, const float* force_field, float* rho,
+                float* u, float* p);
+
+/*
+ * f = feq(rho, u) (lbm.py:180-206) into a state in d->storage.
+ * rho (*G) and u (*G, D) float; rho may be NULL (rho = 1), u may be NULL (u = 0).
+ */
+int lbm_equilibrium(void* stream, const lbm_desc* d, const float* rho,
+                    const float* u, void* f_out);
+
+/*
+ * Layout/dtype conversion between the reference's (*G, Q) array
+ * (lbm.py:18-19) and the SoA state. to_soa != 0: aos -> soa; else soa -> aos.
+ * aos_is_f64 selects float64 (*G, Q) instead of float32 (*G, Q).
+ */
+int lbm_convert(void* stream, const lbm_desc* d, int to_soa, int aos_is_f64,
+                const void* src, void* dst);
+
+/*
+ * FP64 conservation sums of a state: out[0] = sum over cells and q of f,
+ * out[1..D] = sum of f * e_q (raw first moment, no force correction).
+ * out: device double[4]. work: lbm_reduce_workspace_size(d) bytes.
+ */
+size_t lbm_reduce_workspace_size(const lbm_desc* d);
+int lbm_conservation(void* stream, const lbm_desc* d, const void* f,
+                     void* work, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MADDENING_LBM_C_API_H */

@@ -1,0 +1,195 @@
+// Device-side building blocks shared by every LBM kernel: lattice tables,
+// storage traits, the Guo-forced BGK collision, and the row/neighbour index
+// arithmetic of the SoA layout.
+//
+// Internal axes: a state is addressed as [q][ia][ib][ic] with ic contiguous.
+// D3Q19 maps the reference's grid axes (0, 1, 2) to (a, b, c); D2Q9 maps its
+// axes (0, 1) to (b, c) and has na = 1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mlbm {
+
+// ---------------------------------------------------------------------------
+// Lattices. Direction order, opposite map and weights follow the reference's
+// d3q19()/d2q9() (src/maddening/nodes/lbm.py:90-173).
+// ---------------------------------------------------------------------------
+template <int Q> struct Lattice;
+
+template <> struct Lattice<19> {
+  static constexpr int Q = 19;
+  static constexpr int D = 3;
+  static constexpr int ea[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  static constexpr int eb[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+  static constexpr int ec[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+  static constexpr int opp[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+  // weight numerators over 36: 1/3, 1/18 (x6), 1/36 (x12)
+  static constexpr int w36[19] = {12, 2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+};
+
+template <> struct Lattice<9> {
+  static constexpr int Q = 9;
+  static constexpr int D = 2;
+  static constexpr int ea[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  static constexpr int eb[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+  static constexpr int ec[9] = {0, 0, 0, 1, -1, 1, 1, -1, -1};
+  static constexpr int opp[9] = {0, 2, 1, 4, 3, 8, 7, 6, 5};
+  // 4/9, 1/9 (x4), 1/36 (x4)
+  static constexpr int w36[9] = {16, 4, 4, 4, 4, 1, 1, 1, 1};
+};
+
+template <class L, class C>
+__device__ __forceinline__ constexpr C weight(int q) {
+  return C(L::w36[q]) / C(36);
+}
+
+// ---------------------------------------------------------------------------
+// Storage traits. Every kernel works on deviations d_q = f_q - w_q in its
+// compute type C; the trait converts from/to what is stored.
+// ---------------------------------------------------------------------------
+template <class T> struct Storage;
+
+template <> struct Storage<float> {
+  using C = float;
+  template <class L>
+  static __device__ __forceinline__ float load(const float* p, int q) {
+    return *p - weight<L, float>(q);
+  }
+  template <class L>
+  static __device__ __forceinline__ void store(float* p, int q, float d) {
+    __builtin_nontemporal_store(weight<L, float>(q) + d, p);
+  }
+};
+
+template <> struct Storage<double> {
+  using C = double;
+  template <class L>
+  static __device__ __forceinline__ double load(const double* p, int q) {
+    return *p - weight<L, double>(q);
+  }
+  template <class L>
+  static __device__ __forceinline__ void store(double* p, int q, double d) {
+    __builtin_nontemporal_store(weight<L, double>(q) + d, p);
+  }
+};
+
+template <> struct Storage<_Float16> {
+  using C = float;
+  template <class L>
+  static __device__ __forceinline__ float load(const _Float16* p, int) {
+    return (float)(*p);
+  }
+  template <class L>
+  static __device__ __forceinline__ void store(_Float16* p, int, float d) {
+    __builtin_nontemporal_store((_Float16)d, p);
+  }
+};
+
+template <> struct Storage<__bf16> {
+  using C = float;
+  template <class L>
+  static __device__ __forceinline__ float load(const __bf16* p, int) {
+    return (float)(*p);
+  }
+  template <class L>
+  static __device__ __forceinline__ void store(__bf16* p, int, float d) {
+    __bf16 v = (__bf16)d;
+    __builtin_nontemporal_store(v, p);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Macroscopic moments with the Guo half-force correction
+// (lbm.py:209-239): rho = sum f, u = (sum f e + F/2) / max(rho, 1e-10).
+// Computed from deviations: rho = 1 + sum d (sum w = 1), sum f e = sum d e.
+// ---------------------------------------------------------------------------
+template <class L, class C>
+__device__ __forceinline__ void moments(const C (&d)[L::Q], C Fa, C Fb, C Fc,
+                                        C& drho, C& rho, C& ua, C& ub, C& uc) {
+  C s = 0, ja = 0, jb = 0, jc = 0;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    s += d[q];
+    if (L::ea[q] > 0) ja += d[q];
+    if (L::ea[q] < 0) ja -= d[q];
+    if (L::eb[q] > 0) jb += d[q];
+    if (L::eb[q] < 0) jb -= d[q];
+    if (L::ec[q] > 0) jc += d[q];
+    if (L::ec[q] < 0) jc -= d[q];
+  }
+  drho = s;
+  rho = C(1) + s;
+  const C inv = C(1) / (rho > C(1e-10) ? rho : C(1e-10));
+  ua = (ja + C(0.5) * Fa) * inv;
+  ub = (jb + C(0.5) * Fb) * inv;
+  uc = (jc + C(0.5) * Fc) * inv;
+}
+
+// ---------------------------------------------------------------------------
+// BGK collision with the Guo source, in place on deviations
+// (lbm.py:934-939 with _equilibrium lbm.py:180-206, _guo_forcing
+// lbm.py:307-353):
+//   f* = f - (f - feq)/tau + (1 - 1/(2 tau)) w [3 (e - u).F + 9 (e.u)(e.F)]
+//   feq - w = w [ (rho - 1) + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u) ]
+// cs2 = 1/3 gives the exact constants 3, 4.5, 1.5 and 9.
+// ---------------------------------------------------------------------------
+template <class L, class C>
+__device__ __forceinline__ void collide(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
+                                        C Fc, C drho, C rho, C ua, C ub,
+                                        C uc) {
+  const C usq15 = C(1.5) * (ua * ua + ub * ub + uc * uc);
+  const C uF = ua * Fa + ub * Fb + uc * Fc;
+  const C keep = C(1) - inv_tau;
+  const C src = C(1) - C(0.5) * inv_tau;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const C w = weight<L, C>(q);
+    const C eu = C(L::ea[q]) * ua + C(L::eb[q]) * ub + C(L::ec[q]) * uc;
+    const C eF = C(L::ea[q]) * Fa + C(L::eb[q]) * Fb + C(L::ec[q]) * Fc;
+    const C deq = w * (drho + rho * (C(3) * eu + C(4.5) * eu * eu - usq15));
+    const C S = src * w * (C(3) * (eF - uF) + C(9) * eu * eF);
+    d[q] = keep * d[q] + inv_tau * deq + S;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Periodic neighbour rows. For a cell (ia, ib, ic) the index of the cell at
+// offset (sa, sb, sc) in {-1,0,1}^3 is row[sa+1][sb+1] + col[sc+1].
+// ---------------------------------------------------------------------------
+struct Nbr {
+  int row[3][3];
+  int col[3];
+  int cell;
+  __device__ __forceinline__ void init(int ia, int ib, int ic, int na, int nb,
+                                       int nc) {
+    const int A[3] = {ia == 0 ? na - 1 : ia - 1, ia, ia == na - 1 ? 0 : ia + 1};
+    const int B[3] = {ib == 0 ? nb - 1 : ib - 1, ib, ib == nb - 1 ? 0 : ib + 1};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) row[i][j] = (A[i] * nb + B[j]) * nc;
+    col[0] = ic == 0 ? nc - 1 : ic - 1;
+    col[1] = ic;
+    col[2] = ic == nc - 1 ? 0 : ic + 1;
+    cell = row[1][1] + ic;
+  }
+  // index of cell x + s * e_q
+  template <class L>
+  __device__ __forceinline__ int at(int q, int s) const {
+    return row[1 + s * L::ea[q]][1 + s * L::eb[q]] + col[1 + s * L::ec[q]];
+  }
+};
+
+// Map a linear workgroup id to a logical row-block id so that consecutive
+// logical blocks land on the same XCD (workgroups are dealt to the 8 XCDs
+// round-robin; neighbouring rows then share that XCD's L2).
+__device__ __forceinline__ int xcd_remap(int b, int nblocks, bool enable) {
+  if (!enable) return b;
+  const int n8 = (nblocks / 8) * 8;
+  if (b >= n8) return b;
+  return (b % 8) * (n8 / 8) + b / 8;
+}
+
+}  // namespace mlbm

@@ -1,0 +1,706 @@
+// Forward kernels of the LBM step and the C ABI entry points that drive them.
+//
+// One thread owns one cell; a workgroup owns `block_x` consecutive cells of
+// one grid row along the contiguous axis. Every kernel reads each population
+// of its cell once and writes it once, as 19 (or 9) separate coalesced
+// streams, one per population plane.
+//
+// Two ways to chain steps, both reading and writing each population exactly
+// once per step:
+//  * pull: the state kept between kernels is post-collision. One fused kernel
+//    gathers f(x) = g(x - e) (the stream, with bounce-back), which is the
+//    reference's state f after a step, collides it and writes g' at x.
+//    A step sequence is collide-only, (n-1) fused kernels, then a final
+//    stream kernel that also emits the moments.
+//  * push: the state kept between kernels is the reference's f. One kernel
+//    collides f(x) and scatters f*(x) to x + e (or to the bounce-back slot).
+//    Moments come from a separate read-only pass at the end.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/lbm_c_api.h"
+#include "lbm_common.hpp"
+
+namespace mlbm {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+enum { BB_NONE = 0, BB_REF = 1, BB_HALF = 2 };
+
+template <class C>
+struct FwdArgs {
+  const void* src;
+  void* dst;
+  const uint8_t* mask;
+  const float* force_field;
+  const float* tau_field;
+  float* rho;
+  float* u;
+  float* p;
+  long long plane;  // cells per population plane
+  int na, nb, nc;
+  int bpr;          // workgroups per grid row
+  int nblocks;
+  int remap;
+  C inv_tau;
+  C Fa, Fb, Fc;
+  C lid_a, lid_b, lid_c;
+};
+
+template <class L, class C>
+__device__ __forceinline__ void load_force(const FwdArgs<C>& a, int cell, C& Fa,
+                                           C& Fb, C& Fc) {
+  Fa = a.Fa;
+  Fb = a.Fb;
+  Fc = a.Fc;
+  if (a.force_field) {
+    if (L::D == 3) {
+      Fa = C(a.force_field[3 * (long long)cell + 0]);
+      Fb = C(a.force_field[3 * (long long)cell + 1]);
+      Fc = C(a.force_field[3 * (long long)cell + 2]);
+    } else {
+      Fb = C(a.force_field[2 * (long long)cell + 0]);
+      Fc = C(a.force_field[2 * (long long)cell + 1]);
+    }
+  }
+}
+
+template <class L, class C>
+__device__ __forceinline__ void write_moments(const FwdArgs<C>& a, int cell,
+                                              bool wall, C rho, C ua, C ub,
+                                              C uc) {
+  if (a.rho) a.rho[cell] = float(rho);
+  if (a.p) a.p[cell] = float(rho * C(1.0 / 3.0));
+  if (a.u) {
+    if (wall) ua = ub = uc = C(0);
+    if (L::D == 3) {
+      a.u[3 * (long long)cell + 0] = float(ua);
+      a.u[3 * (long long)cell + 1] = float(ub);
+      a.u[3 * (long long)cell + 2] = float(uc);
+    } else {
+      a.u[2 * (long long)cell + 0] = float(ub);
+      a.u[2 * (long long)cell + 1] = float(uc);
+    }
+  }
+}
+
+// 2 w_q (e_q . u_wall) / cs2: momentum a moving wall adds to the population
+// it reflects into direction q (halfway bounce-back, wall density 1).
+template <class L, class C>
+__device__ __forceinline__ C lid_term(const FwdArgs<C>& a, int q) {
+  return C(6) * weight<L, C>(q) *
+         (C(L::ea[q]) * a.lid_a + C(L::eb[q]) * a.lid_b + C(L::ec[q]) * a.lid_c);
+}
+
+// PULL: gather (stream + bounce-back) instead of an aligned read.
+// COLLIDE: apply the collision after the read.
+// STORE: write the state back (false for the moments-only pass).
+template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
+__global__ void __launch_bounds__(256)
+    k_cell(FwdArgs<typename Storage<T>::C> a) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
+  const int row = lb / a.bpr;
+  const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
+  if (ic >= a.nc) return;
+  const int ia = row / a.nb;
+  const int ib = row - ia * a.nb;
+  Nbr n;
+  n.init(ia, ib, ic, a.na, a.nb, a.nc);
+  const T* src = static_cast<const T*>(a.src);
+  const long long P = a.plane;
+  const uint8_t self = a.mask ? a.mask[n.cell] : uint8_t(0);
+
+  C d[L::Q];
+  if (BB == BB_HALF && PULL && self) {
+    // solid cell under halfway bounce-back: no fluid reads it; park it at rest
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) d[q] = C(0);
+  } else {
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+      if (!PULL) {
+        d[q] = St::template load<L>(src + q * P + n.cell, q);
+      } else if (BB == BB_REF) {
+        // reference bounce-back (lbm.py:944-948): at a wall cell slot q takes
+        // the value streamed into slot opp(q), which came from x + e_q.
+        const T* ptr = self ? src + L::opp[q] * P + n.template at<L>(q, +1)
+                            : src + q * P + n.template at<L>(q, -1);
+        d[q] = St::template load<L>(ptr, q);
+      } else if (BB == BB_HALF) {
+        const int s = n.template at<L>(q, -1);
+        const uint8_t m = (q == 0) ? uint8_t(0) : a.mask[s];
+        if (m) {
+          d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) +
+                 (m == 2 ? lid_term<L, C>(a, q) : C(0));
+        } else {
+          d[q] = St::template load<L>(src + q * P + s, q);
+        }
+      } else {
+        d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
+      }
+    }
+  }
+
+  C Fa, Fb, Fc;
+  load_force<L, C>(a, n.cell, Fa, Fb, Fc);
+  C drho, rho, ua, ub, uc;
+  moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+  if (a.rho || a.u || a.p) write_moments<L, C>(a, n.cell, self != 0, rho, ua, ub, uc);
+  if (COLLIDE) {
+    const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
+    collide<L, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+  }
+  if (STORE) {
+    T* dst = static_cast<T*>(a.dst);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) St::template store<L>(dst + q * P + n.cell, q, d[q]);
+  }
+}
+
+// Push: aligned read of the reference state f(x), collision, scatter of f*(x)
+// to x + e_q. Under reference bounce-back a value entering a wall cell lands
+// in the opposite slot (lbm.py:944-948); under halfway bounce-back a value
+// heading into a wall returns to x in the opposite slot.
+template <class L, class T, int BB>
+__global__ void __launch_bounds__(256) k_push(FwdArgs<typename Storage<T>::C> a) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
+  const int row = lb / a.bpr;
+  const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
+  if (ic >= a.nc) return;
+  const int ia = row / a.nb;
+  const int ib = row - ia * a.nb;
+  Nbr n;
+  n.init(ia, ib, ic, a.na, a.nb, a.nc);
+  const T* src = static_cast<const T*>(a.src);
+  T* dst = static_cast<T*>(a.dst);
+  const long long P = a.plane;
+
+  if (BB == BB_HALF && a.mask[n.cell]) {
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) St::template store<L>(dst + q * P + n.cell, q, C(0));
+    return;
+  }
+  C d[L::Q];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) d[q] = St::template load<L>(src + q * P + n.cell, q);
+  C Fa, Fb, Fc;
+  load_force<L, C>(a, n.cell, Fa, Fb, Fc);
+  C drho, rho, ua, ub, uc;
+  moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+  const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
+  collide<L, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int t = n.template at<L>(q, +1);
+    if (BB == BB_NONE) {
+      St::template store<L>(dst + q * P + t, q, d[q]);
+    } else if (BB == BB_REF) {
+      T* ptr = a.mask[t] ? dst + L::opp[q] * P + t : dst + q * P + t;
+      St::template store<L>(ptr, q, d[q]);
+    } else {
+      const uint8_t m = (q == 0) ? uint8_t(0) : a.mask[t];
+      if (m) {
+        St::template store<L>(dst + L::opp[q] * P + n.cell, q,
+                              d[q] - (m == 2 ? lid_term<L, C>(a, q) : C(0)));
+      } else {
+        St::template store<L>(dst + q * P + t, q, d[q]);
+      }
+    }
+  }
+}
+
+// f = feq(rho, u) (lbm.py:180-206) written as deviations.
+template <class L, class T>
+__global__ void __launch_bounds__(256)
+    k_equilibrium(FwdArgs<typename Storage<T>::C> a) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= a.plane) return;
+  const C rho = a.rho ? C(a.rho[cell]) : C(1);
+  C ua = 0, ub = 0, uc = 0;
+  if (a.u) {
+    if (L::D == 3) {
+      ua = C(a.u[3 * cell + 0]);
+      ub = C(a.u[3 * cell + 1]);
+      uc = C(a.u[3 * cell + 2]);
+    } else {
+      ub = C(a.u[2 * cell + 0]);
+      uc = C(a.u[2 * cell + 1]);
+    }
+  }
+  const C drho = rho - C(1);
+  const C usq15 = C(1.5) * (ua * ua + ub * ub + uc * uc);
+  T* dst = static_cast<T*>(a.dst);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const C eu = C(L::ea[q]) * ua + C(L::eb[q]) * ub + C(L::ec[q]) * uc;
+    const C dq = weight<L, C>(q) * (drho + rho * (C(3) * eu + C(4.5) * eu * eu - usq15));
+    St::template store<L>(dst + q * a.plane + cell, q, dq);
+  }
+}
+
+// (*G, Q) float/double <-> SoA storage.
+template <class L, class T, class A, bool TO_SOA>
+__global__ void __launch_bounds__(256) k_convert(FwdArgs<typename Storage<T>::C> a) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= a.plane) return;
+  if (TO_SOA) {
+    const A* aos = static_cast<const A*>(a.src);
+    T* soa = static_cast<T*>(a.dst);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q)
+      St::template store<L>(soa + q * a.plane + cell, q,
+                            C(aos[cell * L::Q + q]) - weight<L, C>(q));
+  } else {
+    const T* soa = static_cast<const T*>(a.src);
+    A* aos = static_cast<A*>(a.dst);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q)
+      aos[cell * L::Q + q] =
+          A(St::template load<L>(soa + q * a.plane + cell, q) + weight<L, C>(q));
+  }
+}
+
+// FP64 conservation sums: per-workgroup partials, then one workgroup folds
+// them (deterministic order).
+constexpr int kReduceBlocks = 1024;
+
+template <class L, class T>
+__global__ void __launch_bounds__(256)
+    k_conserve_partial(const T* f, long long plane, double* partial) {
+  using St = Storage<T>;
+  double s[4] = {0, 0, 0, 0};
+  for (long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x; cell < plane;
+       cell += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+      const double fq = double(St::template load<L>(f + q * plane + cell, q)) +
+                        weight<L, double>(q);
+      s[0] += fq;
+      s[1] += L::ea[q] * fq;
+      s[2] += L::eb[q] * fq;
+      s[3] += L::ec[q] * fq;
+    }
+  }
+  __shared__ double sh[4][256];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) partial[4 * blockIdx.x + k] = sh[k][0];
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_conserve_final(const double* partial, int nparts,
+                                                        double* out) {
+  __shared__ double sh[4][256];
+  double s[4] = {0, 0, 0, 0};
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[k] += partial[4 * i + k];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sh[0][0];
+    if (D == 3) {
+      out[1] = sh[1][0];
+      out[2] = sh[2][0];
+      out[3] = sh[3][0];
+    } else {  // D2Q9 lives on axes (b, c)
+      out[1] = sh[2][0];
+      out[2] = sh[3][0];
+      out[3] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct Geom {
+  int na, nb, nc;
+  long long plane;
+  int bx, bpr, nblocks;
+};
+
+int storage_size(int s) {
+  switch (s) {
+    case LBM_F32: return 4;
+    case LBM_F64: return 8;
+    case LBM_F16_DEV: return 2;
+    case LBM_BF16_DEV: return 2;
+    default: return 0;
+  }
+}
+
+int check_desc(const lbm_desc* d, Geom* g) {
+  if (!d) return fail(LBM_ERR_INVALID, "lbm_desc is NULL");
+  if (d->q != 9 && d->q != 19)
+    return fail(LBM_ERR_INVALID, "q must be 9 (D2Q9) or 19 (D3Q19), got " + std::to_string(d->q));
+  if (storage_size(d->storage) == 0)
+    return fail(LBM_ERR_INVALID, "unknown storage type " + std::to_string(d->storage));
+  if (d->bb_mode < 0 || d->bb_mode > 2)
+    return fail(LBM_ERR_INVALID, "unknown bb_mode " + std::to_string(d->bb_mode));
+  if (d->scheme < 0 || d->scheme > 2)
+    return fail(LBM_ERR_INVALID, "unknown scheme " + std::to_string(d->scheme));
+  if (!(d->tau > 0.5))
+    return fail(LBM_ERR_INVALID, "tau must be > 0.5 (got " + std::to_string(d->tau) + ")");
+  long long n0 = d->dims[0], n1 = d->dims[1], n2 = d->q == 19 ? d->dims[2] : 1;
+  if (n0 < 1 || n1 < 1 || n2 < 1) return fail(LBM_ERR_INVALID, "grid dims must be >= 1");
+  if (d->q == 19) {
+    g->na = (int)n0; g->nb = (int)n1; g->nc = (int)n2;
+  } else {
+    g->na = 1; g->nb = (int)n0; g->nc = (int)n1;
+  }
+  const long long plane = (long long)g->na * g->nb * g->nc;
+  if (plane >= (1LL << 31) || n0 >= (1LL << 31) || n1 >= (1LL << 31) || n2 >= (1LL << 31))
+    return fail(LBM_ERR_UNSUPPORTED, "more than 2^31-1 cells per population plane");
+  g->plane = plane;
+  int bx = d->block_x;
+  if (bx == 0) {
+    bx = g->nc >= 256 ? 256 : ((g->nc + 63) / 64) * 64;
+  }
+  if (bx < 64 || bx > 256 || bx % 64 != 0)
+    return fail(LBM_ERR_INVALID, "block_x must be 64, 128, 192 or 256");
+  g->bx = bx;
+  g->bpr = (g->nc + bx - 1) / bx;
+  const long long nblocks = (long long)g->na * g->nb * g->bpr;
+  if (nblocks >= (1LL << 31)) return fail(LBM_ERR_UNSUPPORTED, "grid too large");
+  g->nblocks = (int)nblocks;
+  return LBM_OK;
+}
+
+template <class C>
+FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
+  FwdArgs<C> a;
+  std::memset(&a, 0, sizeof(a));
+  a.plane = g.plane;
+  a.na = g.na; a.nb = g.nb; a.nc = g.nc;
+  a.bpr = g.bpr;
+  a.nblocks = g.nblocks;
+  a.remap = (d->flags & LBM_FLAG_NO_XCD_REMAP) ? 0 : 1;
+  a.inv_tau = C(1.0 / d->tau);
+  if (d->q == 19) {
+    a.Fa = C(d->force[0]); a.Fb = C(d->force[1]); a.Fc = C(d->force[2]);
+    a.lid_a = C(d->lid_velocity[0]); a.lid_b = C(d->lid_velocity[1]); a.lid_c = C(d->lid_velocity[2]);
+  } else {
+    a.Fa = C(0); a.Fb = C(d->force[0]); a.Fc = C(d->force[1]);
+    a.lid_a = C(0); a.lid_b = C(d->lid_velocity[0]); a.lid_c = C(d->lid_velocity[1]);
+  }
+  return a;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return LBM_OK;
+}
+
+template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
+int launch_cell(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>::C>& a) {
+  hipLaunchKernelGGL((k_cell<L, T, BB, PULL, COLLIDE, STORE>), dim3(g.nblocks), dim3(g.bx), 0, s, a);
+  return check_launch("k_cell launch");
+}
+
+template <class L, class T, int BB>
+int launch_push(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>::C>& a) {
+  hipLaunchKernelGGL((k_push<L, T, BB>), dim3(g.nblocks), dim3(g.bx), 0, s, a);
+  return check_launch("k_push launch");
+}
+
+// Pull sequence: collide (f_in -> B0), fused pull+collide (B -> B'), final
+// pull + moments (B -> f_out). The ping-pong parity puts the last buffer
+// before f_out in `work`.
+template <class L, class T, int BB>
+int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
+              const void* f_in, void* f_out, void* work, int nsteps,
+              float* rho, float* u, float* p) {
+  using C = typename Storage<T>::C;
+  int rc;
+  void* buf[2] = {work, f_out};  // buf[(nsteps-1-i) & 1] holds the state after kernel i
+  FwdArgs<C> k = a;
+  k.rho = k.u = k.p = nullptr;
+  k.src = f_in;
+  k.dst = buf[(nsteps - 1) & 1];
+  if ((rc = launch_cell<L, T, BB, false, true, true>(s, g, k))) return rc;
+  for (int i = 1; i < nsteps; ++i) {
+    k.src = buf[(nsteps - i) & 1];
+    k.dst = buf[(nsteps - 1 - i) & 1];
+    if ((rc = launch_cell<L, T, BB, true, true, true>(s, g, k))) return rc;
+  }
+  k.src = work;
+  k.dst = f_out;
+  k.rho = rho; k.u = u; k.p = p;
+  return launch_cell<L, T, BB, true, false, true>(s, g, k);
+}
+
+template <class L, class T, int BB>
+int step_push(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
+              const void* f_in, void* f_out, void* work, int nsteps,
+              float* rho, float* u, float* p) {
+  using C = typename Storage<T>::C;
+  int rc;
+  void* buf[2] = {f_out, work};  // buf[(nsteps-1-i) & 1]... state after step i+1 -> buf[(nsteps-1-i)&1]
+  FwdArgs<C> k = a;
+  k.rho = k.u = k.p = nullptr;
+  const void* cur = f_in;
+  for (int i = 0; i < nsteps; ++i) {
+    k.src = cur;
+    k.dst = buf[(nsteps - 1 - i) & 1];
+    if ((rc = launch_push<L, T, BB>(s, g, k))) return rc;
+    cur = k.dst;
+  }
+  if (rho || u || p) {
+    k.src = f_out;
+    k.dst = nullptr;
+    k.rho = rho; k.u = u; k.p = p;
+    return launch_cell<L, T, BB_NONE, false, false, false>(s, g, k);
+  }
+  return LBM_OK;
+}
+
+template <class L, class T>
+int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in,
+               void* f_out, void* work, const uint8_t* mask, const float* ff,
+               const float* tf, float* rho, float* u, float* p, int nsteps,
+               int scheme) {
+  using C = typename Storage<T>::C;
+  FwdArgs<C> a = make_args<C>(d, g);
+  a.mask = d->bb_mode == LBM_BB_NONE ? nullptr : mask;
+  a.force_field = ff;
+  a.tau_field = tf;
+  if (scheme == LBM_SCHEME_PULL) {
+    switch (d->bb_mode) {
+      case LBM_BB_NONE: return step_pull<L, T, BB_NONE>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+      case LBM_BB_REFERENCE: return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+      default: return step_pull<L, T, BB_HALF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+    }
+  }
+  switch (d->bb_mode) {
+    case LBM_BB_NONE: return step_push<L, T, BB_NONE>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+    case LBM_BB_REFERENCE: return step_push<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+    default: return step_push<L, T, BB_HALF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+  }
+}
+
+template <template <class, class> class F, class... Args>
+int dispatch(const lbm_desc* d, Args&&... args) {
+  if (d->q == 19) {
+    switch (d->storage) {
+      case LBM_F32: return F<Lattice<19>, float>::run(args...);
+      case LBM_F64: return F<Lattice<19>, double>::run(args...);
+      case LBM_F16_DEV: return F<Lattice<19>, _Float16>::run(args...);
+      default: return F<Lattice<19>, __bf16>::run(args...);
+    }
+  }
+  switch (d->storage) {
+    case LBM_F32: return F<Lattice<9>, float>::run(args...);
+    case LBM_F64: return F<Lattice<9>, double>::run(args...);
+    case LBM_F16_DEV: return F<Lattice<9>, _Float16>::run(args...);
+    default: return F<Lattice<9>, __bf16>::run(args...);
+  }
+}
+
+template <class L, class T>
+struct StepOp {
+  static int run(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in,
+                 void* f_out, void* work, const uint8_t* mask, const float* ff,
+                 const float* tf, float* rho, float* u, float* p, int nsteps, int scheme) {
+    return step_typed<L, T>(s, d, g, f_in, f_out, work, mask, ff, tf, rho, u, p, nsteps, scheme);
+  }
+};
+
+template <class L, class T>
+struct MomentsOp {
+  static int run(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f,
+                 const uint8_t* mask, const float* ff, float* rho, float* u, float* p) {
+    using C = typename Storage<T>::C;
+    FwdArgs<C> a = make_args<C>(d, g);
+    a.src = f;
+    a.mask = mask;
+    a.force_field = ff;
+    a.rho = rho; a.u = u; a.p = p;
+    return launch_cell<L, T, BB_NONE, false, false, false>(s, g, a);
+  }
+};
+
+template <class L, class T>
+struct EquilibriumOp {
+  static int run(hipStream_t s, const lbm_desc* d, const Geom& g, const float* rho,
+                 const float* u, void* f_out) {
+    using C = typename Storage<T>::C;
+    FwdArgs<C> a = make_args<C>(d, g);
+    a.rho = const_cast<float*>(rho);
+    a.u = const_cast<float*>(u);
+    a.dst = f_out;
+    const int nb = (int)((g.plane + 255) / 256);
+    hipLaunchKernelGGL((k_equilibrium<L, T>), dim3(nb), dim3(256), 0, s, a);
+    return check_launch("k_equilibrium launch");
+  }
+};
+
+template <class L, class T>
+struct ConvertOp {
+  static int run(hipStream_t s, const lbm_desc* d, const Geom& g, int to_soa, int f64,
+                 const void* src, void* dst) {
+    using C = typename Storage<T>::C;
+    FwdArgs<C> a = make_args<C>(d, g);
+    a.src = src;
+    a.dst = dst;
+    const int nb = (int)((g.plane + 255) / 256);
+    if (to_soa && f64) hipLaunchKernelGGL((k_convert<L, T, double, true>), dim3(nb), dim3(256), 0, s, a);
+    else if (to_soa) hipLaunchKernelGGL((k_convert<L, T, float, true>), dim3(nb), dim3(256), 0, s, a);
+    else if (f64) hipLaunchKernelGGL((k_convert<L, T, double, false>), dim3(nb), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_convert<L, T, float, false>), dim3(nb), dim3(256), 0, s, a);
+    return check_launch("k_convert launch");
+  }
+};
+
+template <class L, class T>
+struct ConserveOp {
+  static int run(hipStream_t s, const lbm_desc*, const Geom& g, const void* f, void* work,
+                 double* out) {
+    double* partial = static_cast<double*>(work);
+    long long nb = (g.plane + 255) / 256;
+    const int nparts = (int)(nb < kReduceBlocks ? nb : kReduceBlocks);
+    hipLaunchKernelGGL((k_conserve_partial<L, T>), dim3(nparts), dim3(256), 0, s,
+                       static_cast<const T*>(f), g.plane, partial);
+    int rc = check_launch("k_conserve_partial launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_conserve_final<L::D>), dim3(1), dim3(256), 0, s,
+                       (const double*)partial, nparts, out);
+    return check_launch("k_conserve_final launch");
+  }
+};
+
+}  // namespace mlbm
+
+using namespace mlbm;
+
+extern "C" {
+
+int lbm_version(void) { return LBM_API_VERSION; }
+
+const char* lbm_last_error(void) { return g_last_error.c_str(); }
+
+size_t lbm_state_bytes(const lbm_desc* d) {
+  Geom g;
+  if (check_desc(d, &g)) return 0;
+  return (size_t)g.plane * (size_t)d->q * (size_t)storage_size(d->storage);
+}
+
+static int resolve_scheme(const lbm_desc* d) {
+  if (d->scheme != LBM_SCHEME_AUTO) return d->scheme;
+  return LBM_SCHEME_PULL;
+}
+
+size_t lbm_workspace_size(const lbm_desc* d, int nsteps) {
+  Geom g;
+  if (check_desc(d, &g)) return 0;
+  const int scheme = resolve_scheme(d);
+  if (scheme == LBM_SCHEME_PUSH && nsteps <= 1) return 0;
+  return (size_t)g.plane * (size_t)d->q * (size_t)storage_size(d->storage);
+}
+
+int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out, void* work,
+             const uint8_t* mask, const float* force_field, const float* tau_field, float* rho,
+             float* u, float* p, int nsteps) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (nsteps < 1) return fail(LBM_ERR_INVALID, "nsteps must be >= 1");
+  if (!f_in || !f_out) return fail(LBM_ERR_INVALID, "f_in and f_out must be non-NULL");
+  if (f_in == f_out) return fail(LBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
+  if (d->bb_mode != LBM_BB_NONE && !mask)
+    return fail(LBM_ERR_INVALID, "bb_mode needs a wall mask");
+  const int scheme = resolve_scheme(d);
+  if (lbm_workspace_size(d, nsteps) > 0 && !work)
+    return fail(LBM_ERR_INVALID, "this step needs a workspace of lbm_workspace_size() bytes");
+  if (work && (work == f_in || work == f_out))
+    return fail(LBM_ERR_INVALID, "work must not alias f_in or f_out");
+  return dispatch<StepOp>(d, (hipStream_t)stream, d, g, f_in, f_out, work, mask, force_field,
+                          tau_field, rho, u, p, nsteps, scheme);
+}
+
+int lbm_moments(void* stream, const lbm_desc* d, const void* f, const uint8_t* mask,
+                const float* force_field, float* rho, float* u, float* p) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (!f) return fail(LBM_ERR_INVALID, "f must be non-NULL");
+  return dispatch<MomentsOp>(d, (hipStream_t)stream, d, g, f, mask, force_field, rho, u, p);
+}
+
+int lbm_equilibrium(void* stream, const lbm_desc* d, const float* rho, const float* u,
+                    void* f_out) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (!f_out) return fail(LBM_ERR_INVALID, "f_out must be non-NULL");
+  return dispatch<EquilibriumOp>(d, (hipStream_t)stream, d, g, rho, u, f_out);
+}
+
+int lbm_convert(void* stream, const lbm_desc* d, int to_soa, int aos_is_f64, const void* src,
+                void* dst) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (!src || !dst) return fail(LBM_ERR_INVALID, "src and dst must be non-NULL");
+  if (src == dst) return fail(LBM_ERR_INVALID, "src and dst must be distinct buffers");
+  return dispatch<ConvertOp>(d, (hipStream_t)stream, d, g, to_soa, aos_is_f64, src, dst);
+}
+
+size_t lbm_reduce_workspace_size(const lbm_desc*) {
+  return sizeof(double) * 4 * kReduceBlocks;
+}
+
+int lbm_conservation(void* stream, const lbm_desc* d, const void* f, void* work, double* out) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (!f || !work || !out) return fail(LBM_ERR_INVALID, "f, work and out must be non-NULL");
+  return dispatch<ConserveOp>(d, (hipStream_t)stream, d, g, f, work, out);
+}
+
+}  // extern "C"
