@@ -74,12 +74,16 @@ typedef struct lbm_desc {
   double tau;           /* BGK relaxation time used where tau_field is NULL; must be > 0.5 */
   double force[3];      /* uniform body force used where force_field is NULL */
   double lid_velocity[3]; /* velocity of mask==2 cells in LBM_BB_HALFWAY */
-  int32_t block_x;      /* threads per block along the contiguous axis; 0 = auto */
+  int32_t block_x;      /* threads per workgroup of the one-cell-per-thread kernels; 0 = auto */
+  int32_t row_vec;      /* periodic lattices only: cells per lane of the row-per-wave kernels
+                           (1, 2 or 4; the contiguous axis must be a multiple of 64*row_vec);
+                           0 = auto, -1 = never */
   int32_t flags;        /* LBM_FLAG_* */
+  int32_t reserved;     /* must be 0 */
 } lbm_desc;
 
 /* flags */
-#define LBM_FLAG_NO_XCD_REMAP 1  /* keep hardware workgroup order (default remaps rows per XCD) */
+#define LBM_FLAG_XCD_REMAP 1  /* deal consecutive rows to one XCD (off by default: measured slower) */
 
 /* ---- queries ----------------------------------------------------------- */
 

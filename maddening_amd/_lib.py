@@ -17,7 +17,7 @@ from .build import library_path
 LBM_F32, LBM_F64, LBM_F16_DEV, LBM_BF16_DEV = 0, 1, 2, 3
 LBM_BB_NONE, LBM_BB_REFERENCE, LBM_BB_HALFWAY = 0, 1, 2
 LBM_SCHEME_AUTO, LBM_SCHEME_PULL, LBM_SCHEME_PUSH = 0, 1, 2
-LBM_FLAG_NO_XCD_REMAP = 1
+LBM_FLAG_XCD_REMAP = 1
 
 STORAGE_NAMES = {"f32": LBM_F32, "f64": LBM_F64, "f16dev": LBM_F16_DEV,
                  "bf16dev": LBM_BF16_DEV}
@@ -38,7 +38,9 @@ class LbmDesc(ctypes.Structure):
         ("force", ctypes.c_double * 3),
         ("lid_velocity", ctypes.c_double * 3),
         ("block_x", ctypes.c_int32),
+        ("row_vec", ctypes.c_int32),
         ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -1,0 +1,460 @@
+// Discrete adjoint of the LBM step (reverse mode through one
+// collide -> stream -> bounce-back update of lbm.py:899-986).
+//
+// One step is f' = B S C(f): C the per-cell Guo-forced BGK collision, S the
+// periodic shift, B the wall rule. Its transpose applied to a cotangent f'_bar:
+//   g_bar = S^T B^T f'_bar   a gather from x + e_p (the forward pull reversed)
+//   f_bar = C'(f)^T g_bar    local: (1 - 1/tau) g_bar_p + A + B . e_p, where
+//                            A, B come from the cotangents of rho and u that
+//                            feq and the Guo source pass back.
+// The same pass accumulates dL/dtau(x) and dL/dF (per cell, or per workgroup
+// partials for a uniform force). Gradient buffers are plain values (not
+// deviations) in the compute type: float for f32 storage, double for f64.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/lbm_c_api.h"
+#include "lbm_common.hpp"
+#include "lbm_host.hpp"
+
+namespace mlbm {
+
+template <class C>
+struct AdjArgs {
+  const void* f;        // state f_n the step started from (storage type)
+  const C* gin;         // dL/df_{n+1}
+  C* gout;              // dL/df_n
+  const uint8_t* mask;
+  const float* force_field;
+  const float* tau_field;
+  float* tau_bar;       // += dL/dtau(x)
+  float* force_bar;     // += dL/dF(x), (*G, D)
+  double* partial;      // += per-workgroup dL/dF for a uniform force, [nblocks][4]
+  const float* rho_bar; // moments cotangents (moments adjoint only)
+  const float* u_bar;
+  const float* p_bar;
+  long long plane;
+  int na, nb, nc;
+  int bpr, nblocks, remap;
+  C inv_tau;
+  C Fa, Fb, Fc;
+};
+
+// Sum three values over the workgroup and add them to partial[4*lb + k].
+template <int NT>
+__device__ __forceinline__ void block_accumulate(double v0, double v1, double v2, double* partial,
+                                                 int lb) {
+  __shared__ double sh[3][NT];
+  sh[0][threadIdx.x] = v0;
+  sh[1][threadIdx.x] = v1;
+  sh[2][threadIdx.x] = v2;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+      sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[4 * (long long)lb + 0] += sh[0][0];
+    partial[4 * (long long)lb + 1] += sh[1][0];
+    partial[4 * (long long)lb + 2] += sh[2][0];
+  }
+}
+
+template <class L, class C>
+__device__ __forceinline__ void add_force_bar(const AdjArgs<C>& a, int cell, C ga, C gb, C gc) {
+  if (L::D == 3) {
+    a.force_bar[3 * (long long)cell + 0] += float(ga);
+    a.force_bar[3 * (long long)cell + 1] += float(gb);
+    a.force_bar[3 * (long long)cell + 2] += float(gc);
+  } else {
+    a.force_bar[2 * (long long)cell + 0] += float(gb);
+    a.force_bar[2 * (long long)cell + 1] += float(gc);
+  }
+}
+
+template <class L, class C>
+__device__ __forceinline__ void read_force(const AdjArgs<C>& a, int cell, C& Fa, C& Fb, C& Fc) {
+  Fa = a.Fa;
+  Fb = a.Fb;
+  Fc = a.Fc;
+  if (a.force_field) {
+    if (L::D == 3) {
+      Fa = C(a.force_field[3 * (long long)cell + 0]);
+      Fb = C(a.force_field[3 * (long long)cell + 1]);
+      Fc = C(a.force_field[3 * (long long)cell + 2]);
+    } else {
+      Fb = C(a.force_field[2 * (long long)cell + 0]);
+      Fc = C(a.force_field[2 * (long long)cell + 1]);
+    }
+  }
+}
+
+template <class L, class T, int BB>
+__global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C> a) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
+  const int row = lb / a.bpr;
+  const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
+  const bool valid = ic < a.nc;
+  C dFa = 0, dFb = 0, dFc = 0;
+  if (valid) {
+    const int ia = row / a.nb;
+    const int ib = row - ia * a.nb;
+    Nbr n;
+    n.init(ia, ib, ic, a.na, a.nb, a.nc);
+    const long long P = a.plane;
+    const uint8_t self = a.mask ? a.mask[n.cell] : uint8_t(0);
+    if (BB == BB_HALF && self) {
+      // a solid cell is reset to rest every step: nothing flows back into it
+#pragma unroll
+      for (int q = 0; q < L::Q; ++q) a.gout[q * P + n.cell] = C(0);
+    } else {
+      // g_bar = S^T B^T f'_bar: the value the forward step moved out of
+      // slot p at this cell went to x = cell + e_p (slot p, or slot opp(p) if
+      // x is a wall under the reference rule); under halfway bounce-back a
+      // population blocked by a wall came back to this cell in slot opp(p).
+      C gb[L::Q];
+#pragma unroll
+      for (int p = 0; p < L::Q; ++p) {
+        const int x = n.template at<L>(p, +1);
+        if (BB == BB_NONE) {
+          gb[p] = a.gin[p * P + x];
+        } else if (BB == BB_REF) {
+          gb[p] = a.gin[(a.mask[x] ? L::opp[p] : p) * P + x];
+        } else {
+          const bool hit = (p != 0) && a.mask[x] != 0;
+          gb[p] = hit ? a.gin[L::opp[p] * P + n.cell] : a.gin[p * P + x];
+        }
+      }
+      // forward quantities of this cell
+      const T* f = static_cast<const T*>(a.f);
+      C d[L::Q];
+#pragma unroll
+      for (int q = 0; q < L::Q; ++q) d[q] = St::template load<L>(f + q * P + n.cell, q);
+      C Fa, Fb, Fc;
+      read_force<L, C>(a, n.cell, Fa, Fb, Fc);
+      C drho, rho, ua, ub, uc;
+      moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      const C om = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
+      const C c = C(1) - C(0.5) * om;
+      const C usq = ua * ua + ub * ub + uc * uc;
+      const C uF = ua * Fa + ub * Fb + uc * Fc;
+      // sums over q of g_bar_q w_q times {1, e.u, (e.u)^2, e_a, (e.u) e_a, (e.F) e_a}
+      C G0 = 0, Geu = 0, Geu2 = 0, Ga = 0, Gb = 0, Gc = 0;
+      C Ha = 0, Hb = 0, Hc = 0;  // (3 om rho e.u + 9/... ) collected below
+      C Ka = 0, Kb = 0, Kc = 0;  // g w (e.u) e
+      C tb = 0;                   // sum g_bar (dC/domega) for tau
+#pragma unroll
+      for (int q = 0; q < L::Q; ++q) {
+        const C w = weight<L, C>(q);
+        const C gw = gb[q] * w;
+        const C eu = C(L::ea[q]) * ua + C(L::eb[q]) * ub + C(L::ec[q]) * uc;
+        const C eF = C(L::ea[q]) * Fa + C(L::eb[q]) * Fb + C(L::ec[q]) * Fc;
+        G0 += gw;
+        Geu += gw * eu;
+        Geu2 += gw * eu * eu;
+        if (L::ea[q]) { Ga += L::ea[q] * gw; Ka += L::ea[q] * gw * eu; Ha += L::ea[q] * gw * eF; }
+        if (L::eb[q]) { Gb += L::eb[q] * gw; Kb += L::eb[q] * gw * eu; Hb += L::eb[q] * gw * eF; }
+        if (L::ec[q]) { Gc += L::ec[q] * gw; Kc += L::ec[q] * gw * eu; Hc += L::ec[q] * gw * eF; }
+        // d f*_q / d omega = (feq_q - f_q) - 0.5 w [3 (e.F - u.F) + 9 (e.u)(e.F)]
+        const C deq = w * (drho + rho * (C(3) * eu + C(4.5) * eu * eu - C(1.5) * usq));
+        const C sraw = w * (C(3) * (eF - uF) + C(9) * eu * eF);
+        tb += gb[q] * (deq - d[q] - C(0.5) * sraw);
+      }
+      // cotangents of rho (u held fixed) and u from feq (times omega) and the source (times c)
+      const C rho_bar = om * (G0 * (C(1) - C(1.5) * usq) + C(3) * Geu + C(4.5) * Geu2);
+      const C orho = om * rho;
+      const C ubar_a = orho * (C(3) * Ga + C(9) * Ka - C(3) * ua * G0) + c * (C(9) * Ha - C(3) * Fa * G0);
+      const C ubar_b = orho * (C(3) * Gb + C(9) * Kb - C(3) * ub * G0) + c * (C(9) * Hb - C(3) * Fb * G0);
+      const C ubar_c = orho * (C(3) * Gc + C(9) * Kc - C(3) * uc * G0) + c * (C(9) * Hc - C(3) * Fc * G0);
+      // u = (j + F/2) / rho: du/df_p = (e_p - u)/rho, drho/df_p = 1
+      const C inv_rho = C(1) / (rho > C(1e-10) ? rho : C(1e-10));
+      const C Ba = ubar_a * inv_rho, Bb = ubar_b * inv_rho, Bc = ubar_c * inv_rho;
+      const C A = rho_bar - (Ba * ua + Bb * ub + Bc * uc);
+      const C keep = C(1) - om;
+#pragma unroll
+      for (int p = 0; p < L::Q; ++p)
+        a.gout[p * P + n.cell] = keep * gb[p] + A + C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
+      if (a.tau_bar) a.tau_bar[n.cell] += float(-om * om * tb);
+      // dL/dF: the source directly (u fixed) plus through u = (j + F/2)/rho
+      dFa = c * (C(3) * Ga - C(3) * ua * G0 + C(9) * Ka) + C(0.5) * Ba;
+      dFb = c * (C(3) * Gb - C(3) * ub * G0 + C(9) * Kb) + C(0.5) * Bb;
+      dFc = c * (C(3) * Gc - C(3) * uc * G0 + C(9) * Kc) + C(0.5) * Bc;
+      if (a.force_bar) add_force_bar<L, C>(a, n.cell, dFa, dFb, dFc);
+    }
+  }
+  if (a.partial) block_accumulate<256>(double(dFa), double(dFb), double(dFc), a.partial, lb);
+}
+
+// Cotangent of the output moments (lbm.py:970-976) pulled back onto f:
+// f_bar_p += rho_bar + p_bar/3 + sum_a u_bar_a (e_pa - u_a)/rho (u_bar = 0
+// at wall cells, where u is set to 0), and dL/dF += u_bar / (2 rho).
+template <class L, class T>
+__global__ void __launch_bounds__(256) k_moments_adjoint(AdjArgs<typename Storage<T>::C> a) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  const int lb = blockIdx.x;
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  C dFa = 0, dFb = 0, dFc = 0;
+  if (cell < a.plane) {
+    const long long P = a.plane;
+    const T* f = static_cast<const T*>(a.f);
+    C d[L::Q];
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) d[q] = St::template load<L>(f + q * P + cell, q);
+    C Fa, Fb, Fc;
+    read_force<L, C>(a, (int)cell, Fa, Fb, Fc);
+    C drho, rho, ua, ub, uc;
+    moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+    C rb = a.rho_bar ? C(a.rho_bar[cell]) : C(0);
+    if (a.p_bar) rb += C(a.p_bar[cell]) * C(1.0 / 3.0);
+    C va = 0, vb = 0, vc = 0;
+    const bool wall = a.mask && a.mask[cell];
+    if (a.u_bar && !wall) {
+      if (L::D == 3) {
+        va = C(a.u_bar[3 * cell + 0]);
+        vb = C(a.u_bar[3 * cell + 1]);
+        vc = C(a.u_bar[3 * cell + 2]);
+      } else {
+        vb = C(a.u_bar[2 * cell + 0]);
+        vc = C(a.u_bar[2 * cell + 1]);
+      }
+    }
+    const C inv_rho = C(1) / (rho > C(1e-10) ? rho : C(1e-10));
+    const C Ba = va * inv_rho, Bb = vb * inv_rho, Bc = vc * inv_rho;
+    const C A = rb - (Ba * ua + Bb * ub + Bc * uc);
+#pragma unroll
+    for (int p = 0; p < L::Q; ++p) {
+      const C base = a.gin ? a.gin[p * P + cell] : C(0);
+      a.gout[p * P + cell] = base + A + C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
+    }
+    dFa = C(0.5) * Ba;
+    dFb = C(0.5) * Bb;
+    dFc = C(0.5) * Bc;
+    if (a.force_bar) add_force_bar<L, C>(a, (int)cell, dFa, dFb, dFc);
+  }
+  if (a.partial) block_accumulate<256>(double(dFa), double(dFb), double(dFc), a.partial, lb);
+}
+
+__global__ void __launch_bounds__(256) k_fill_zero(double* p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_partial_sum(const double* partial, long long nparts,
+                                                     double* out) {
+  __shared__ double sh[3][256];
+  double s[3] = {0, 0, 0};
+  for (long long i = threadIdx.x; i < nparts; i += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] += partial[4 * i + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (D == 3) {
+      out[0] = sh[0][0];
+      out[1] = sh[1][0];
+      out[2] = sh[2][0];
+    } else {  // D2Q9 axes (b, c) are the reference's (0, 1)
+      out[0] = sh[1][0];
+      out[1] = sh[2][0];
+      out[2] = 0.0;
+    }
+  }
+}
+
+// ---- host ------------------------------------------------------------------
+// Adjoint launches use 256-thread workgroups (the block reduction's width).
+inline int adj_geom(const lbm_desc* d, Geom* g) {
+  int rc = check_desc(d, g);
+  if (rc) return rc;
+  if (d->storage != LBM_F32 && d->storage != LBM_F64)
+    return fail(LBM_ERR_UNSUPPORTED, "the adjoint needs f32 or f64 storage");
+  g->bx = 256;
+  g->bpr = (g->nc + 255) / 256;
+  g->nblocks = g->na * g->nb * g->bpr;
+  return LBM_OK;
+}
+
+inline long long partial_slots(const Geom& g) {
+  const long long cell_blocks = (g.plane + 255) / 256;
+  return g.nblocks > cell_blocks ? g.nblocks : cell_blocks;
+}
+
+template <class C>
+AdjArgs<C> adj_args(const lbm_desc* d, const Geom& g) {
+  AdjArgs<C> a = {};
+  a.plane = g.plane;
+  a.na = g.na; a.nb = g.nb; a.nc = g.nc;
+  a.bpr = g.bpr;
+  a.nblocks = g.nblocks;
+  a.remap = (d->flags & LBM_FLAG_XCD_REMAP) ? 1 : 0;
+  a.inv_tau = C(1.0 / d->tau);
+  if (d->q == 19) {
+    a.Fa = C(d->force[0]); a.Fb = C(d->force[1]); a.Fc = C(d->force[2]);
+  } else {
+    a.Fa = C(0); a.Fb = C(d->force[0]); a.Fc = C(d->force[1]);
+  }
+  return a;
+}
+
+template <class L, class T>
+int adjoint_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f,
+                  const void* gin, void* gout, const uint8_t* mask, const float* ff,
+                  const float* tf, float* tau_bar, float* force_bar, void* work) {
+  using C = typename Storage<T>::C;
+  AdjArgs<C> a = adj_args<C>(d, g);
+  a.f = f;
+  a.gin = static_cast<const C*>(gin);
+  a.gout = static_cast<C*>(gout);
+  a.mask = d->bb_mode == LBM_BB_NONE ? nullptr : mask;
+  a.force_field = ff;
+  a.tau_field = tf;
+  a.tau_bar = tau_bar;
+  a.force_bar = force_bar;
+  a.partial = static_cast<double*>(work);
+  switch (d->bb_mode) {
+    case LBM_BB_NONE:
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_NONE>), dim3(g.nblocks), dim3(256), 0, s, a);
+      break;
+    case LBM_BB_REFERENCE:
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_REF>), dim3(g.nblocks), dim3(256), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF>), dim3(g.nblocks), dim3(256), 0, s, a);
+  }
+  return check_launch("k_adjoint launch");
+}
+
+template <class L, class T>
+int moments_adjoint_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f,
+                          const uint8_t* mask, const float* ff, const float* rho_bar,
+                          const float* u_bar, const float* p_bar, const void* gin, void* gout,
+                          float* force_bar, void* work) {
+  using C = typename Storage<T>::C;
+  AdjArgs<C> a = adj_args<C>(d, g);
+  a.f = f;
+  a.mask = mask;
+  a.force_field = ff;
+  a.rho_bar = rho_bar;
+  a.u_bar = u_bar;
+  a.p_bar = p_bar;
+  a.gin = static_cast<const C*>(gin);
+  a.gout = static_cast<C*>(gout);
+  a.force_bar = force_bar;
+  a.partial = static_cast<double*>(work);
+  const long long nb = (g.plane + 255) / 256;
+  hipLaunchKernelGGL((k_moments_adjoint<L, T>), dim3((unsigned)nb), dim3(256), 0, s, a);
+  return check_launch("k_moments_adjoint launch");
+}
+
+}  // namespace mlbm
+
+using namespace mlbm;
+
+extern "C" {
+
+size_t lbm_adjoint_workspace_size(const lbm_desc* d) {
+  Geom g;
+  if (adj_geom(d, &g)) return 0;
+  return (size_t)partial_slots(g) * 4 * sizeof(double);
+}
+
+int lbm_adjoint_begin(void* stream, const lbm_desc* d, void* work) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = adj_geom(d, &g);
+  if (rc) return rc;
+  if (!work) return fail(LBM_ERR_INVALID, "work must be non-NULL");
+  const long long n = partial_slots(g) * 4;
+  hipLaunchKernelGGL(k_fill_zero, dim3(256), dim3(256), 0, (hipStream_t)stream,
+                     static_cast<double*>(work), n);
+  return check_launch("k_fill_zero launch");
+}
+
+int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, const void* fbar_next,
+                     void* fbar_out, const uint8_t* mask, const float* force_field,
+                     const float* tau_field, float* tau_bar, float* force_bar_field,
+                     void* work) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = adj_geom(d, &g);
+  if (rc) return rc;
+  if (!f_n || !fbar_next || !fbar_out)
+    return fail(LBM_ERR_INVALID, "f_n, fbar_next and fbar_out must be non-NULL");
+  if (fbar_next == fbar_out) return fail(LBM_ERR_INVALID, "fbar_next and fbar_out must be distinct");
+  if (d->bb_mode != LBM_BB_NONE && !mask) return fail(LBM_ERR_INVALID, "bb_mode needs a wall mask");
+  hipStream_t s = (hipStream_t)stream;
+  if (d->q == 19) {
+    if (d->storage == LBM_F32)
+      return adjoint_typed<Lattice<19>, float>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+                                              tau_field, tau_bar, force_bar_field, work);
+    return adjoint_typed<Lattice<19>, double>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+                                             tau_field, tau_bar, force_bar_field, work);
+  }
+  if (d->storage == LBM_F32)
+    return adjoint_typed<Lattice<9>, float>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+                                           tau_field, tau_bar, force_bar_field, work);
+  return adjoint_typed<Lattice<9>, double>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+                                          tau_field, tau_bar, force_bar_field, work);
+}
+
+int lbm_moments_adjoint(void* stream, const lbm_desc* d, const void* f, const uint8_t* mask,
+                        const float* force_field, const float* rho_bar, const float* u_bar,
+                        const float* p_bar, const void* fbar_in, void* fbar_out,
+                        float* force_bar_field, void* work) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = adj_geom(d, &g);
+  if (rc) return rc;
+  if (!f || !fbar_out) return fail(LBM_ERR_INVALID, "f and fbar_out must be non-NULL");
+  hipStream_t s = (hipStream_t)stream;
+  if (d->q == 19) {
+    if (d->storage == LBM_F32)
+      return moments_adjoint_typed<Lattice<19>, float>(s, d, g, f, mask, force_field, rho_bar, u_bar,
+                                                      p_bar, fbar_in, fbar_out, force_bar_field, work);
+    return moments_adjoint_typed<Lattice<19>, double>(s, d, g, f, mask, force_field, rho_bar, u_bar,
+                                                     p_bar, fbar_in, fbar_out, force_bar_field, work);
+  }
+  if (d->storage == LBM_F32)
+    return moments_adjoint_typed<Lattice<9>, float>(s, d, g, f, mask, force_field, rho_bar, u_bar,
+                                                   p_bar, fbar_in, fbar_out, force_bar_field, work);
+  return moments_adjoint_typed<Lattice<9>, double>(s, d, g, f, mask, force_field, rho_bar, u_bar,
+                                                  p_bar, fbar_in, fbar_out, force_bar_field, work);
+}
+
+int lbm_adjoint_end(void* stream, const lbm_desc* d, const void* work, double* force_bar_out) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = adj_geom(d, &g);
+  if (rc) return rc;
+  if (!work || !force_bar_out) return fail(LBM_ERR_INVALID, "work and force_bar_out must be non-NULL");
+  if (d->q == 19)
+    hipLaunchKernelGGL(k_partial_sum<3>, dim3(1), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<const double*>(work), partial_slots(g), force_bar_out);
+  else
+    hipLaunchKernelGGL(k_partial_sum<2>, dim3(1), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<const double*>(work), partial_slots(g), force_bar_out);
+  return check_launch("k_partial_sum launch");
+}
+
+}  // extern "C"

@@ -23,17 +23,10 @@
 
 #include "../../include/lbm_c_api.h"
 #include "lbm_common.hpp"
+#include "lbm_host.hpp"
 
 namespace mlbm {
 
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string& msg) {
-  g_last_error = msg;
-  return code;
-}
-
-enum { BB_NONE = 0, BB_REF = 1, BB_HALF = 2 };
 
 template <class C>
 struct FwdArgs {
@@ -221,6 +214,148 @@ __global__ void __launch_bounds__(256) k_push(FwdArgs<typename Storage<T>::C> a)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Row-per-wave variant for periodic lattices whose contiguous axis is a
+// multiple of 64*V cells. A wave owns 64*V consecutive cells of one row (lane
+// l holds cells V*l .. V*l+V-1), so every population plane is read and written
+// with V-wide vector accesses, and the row offsets of the 9 (a, b)
+// neighbours are wave-uniform. The +-1 shift along the row is a lane rotation
+// (DPP wave_ror / wave_rol); when a row spans several waves, the one value
+// that crosses a wave boundary comes from a single-lane load.
+// ---------------------------------------------------------------------------
+template <class T, int V> struct VecT { typedef T type __attribute__((ext_vector_type(V))); };
+template <class T> struct VecT<T, 1> { typedef T type; };
+
+template <class T, int V>
+__device__ __forceinline__ void vload(const T* p, T (&out)[V]) {
+  using VT = typename VecT<T, V>::type;
+  const VT v = *reinterpret_cast<const VT*>(p);
+  if constexpr (V == 1) {
+    out[0] = v;
+  } else {
+#pragma unroll
+    for (int k = 0; k < V; ++k) out[k] = v[k];
+  }
+}
+
+template <class T, int V>
+__device__ __forceinline__ void vstore_nt(T* p, const T (&in)[V]) {
+  using VT = typename VecT<T, V>::type;
+  VT v;
+  if constexpr (V == 1) {
+    v = in[0];
+  } else {
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] = in[k];
+  }
+  __builtin_nontemporal_store(v, reinterpret_cast<VT*>(p));
+}
+
+// lane i receives lane i-1 (lane 0 receives lane 63): DPP wave_ror:1
+__device__ __forceinline__ float lane_from_prev(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x13C, 0xF, 0xF, false));
+}
+// lane i receives lane i+1 (lane 63 receives lane 0): DPP wave_rol:1
+__device__ __forceinline__ float lane_from_next(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_from_prev(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x13C, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x13C, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_from_next(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x134, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x134, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <class L, class T, int V, bool PULL, bool COLLIDE>
+__global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  constexpr int SEG = 64 * V;
+  const int lane = threadIdx.x & 63;
+  const int segs = a.nc / SEG;  // wave segments per row
+  const long long wave = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const long long nrows = (long long)a.na * a.nb;
+  if (wave >= nrows * segs) return;  // whole wave leaves together
+  const int row = (int)(wave / segs);
+  const int seg = (int)(wave - (long long)row * segs);
+  const int ia = row / a.nb;
+  const int ib = row - ia * a.nb;
+  const int A[3] = {ia == 0 ? a.na - 1 : ia - 1, ia, ia == a.na - 1 ? 0 : ia + 1};
+  const int B[3] = {ib == 0 ? a.nb - 1 : ib - 1, ib, ib == a.nb - 1 ? 0 : ib + 1};
+  const int c0 = seg * SEG;       // first cell of this wave's segment
+  const int cl = c0 + lane * V;   // first cell of this lane
+  // neighbour segment edges along the row (periodic)
+  const int c_before = c0 == 0 ? a.nc - 1 : c0 - 1;
+  const int c_after = c0 + SEG == a.nc ? 0 : c0 + SEG;
+  const long long P = a.plane;
+  const T* src = static_cast<const T*>(a.src);
+
+  C d[L::Q][V];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    // pull from x - e_q: source row (ia - ea, ib - eb)
+    const int sr = PULL ? (A[1 - L::ea[q]] * a.nb + B[1 - L::eb[q]]) : row;
+    const T* rowp = src + q * P + (long long)sr * a.nc;
+    T raw[V];
+    vload<T, V>(rowp + cl, raw);
+    C v[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] = St::template load<L>(&raw[k], q);
+    if (!PULL || L::ec[q] == 0) {
+#pragma unroll
+      for (int k = 0; k < V; ++k) d[q][k] = v[k];
+    } else if (L::ec[q] > 0) {
+      // cell c takes c - 1: lane's first cell takes the previous lane's last
+      C edge = lane_from_prev(v[V - 1]);
+      if (segs > 1 && lane == 0) edge = St::template load<L>(rowp + c_before, q);
+      d[q][0] = edge;
+#pragma unroll
+      for (int k = 1; k < V; ++k) d[q][k] = v[k - 1];
+    } else {
+      // cell c takes c + 1: lane's last cell takes the next lane's first
+      C edge = lane_from_next(v[0]);
+      if (segs > 1 && lane == 63) edge = St::template load<L>(rowp + c_after, q);
+#pragma unroll
+      for (int k = 0; k < V - 1; ++k) d[q][k] = v[k + 1];
+      d[q][V - 1] = edge;
+    }
+  }
+
+  const long long cell0 = (long long)row * a.nc + cl;
+  T out[L::Q][V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    C dk[L::Q];
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) dk[q] = d[q][k];
+    const int cell = (int)(cell0 + k);
+    C Fa, Fb, Fc;
+    load_force<L, C>(a, cell, Fa, Fb, Fc);
+    C drho, rho, ua, ub, uc;
+    moments<L, C>(dk, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+    if (a.rho || a.u || a.p) write_moments<L, C>(a, cell, false, rho, ua, ub, uc);
+    if (COLLIDE) {
+      const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[cell]) : a.inv_tau;
+      collide<L, C>(dk, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+    }
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+      // same conversion as Storage<T>::store, kept in registers for a vector store
+      if constexpr (sizeof(T) >= 4) out[q][k] = T(weight<L, C>(q) + dk[q]);
+      else out[q][k] = T(dk[q]);
+    }
+  }
+  T* dst = static_cast<T*>(a.dst);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) vstore_nt<T, V>(dst + q * P + cell0, out[q]);
+}
+
 // f = feq(rho, u) (lbm.py:180-206) written as deviations.
 template <class L, class T>
 __global__ void __launch_bounds__(256)
@@ -347,59 +482,6 @@ __global__ void __launch_bounds__(256) k_conserve_final(const double* partial, i
 // Host side
 // ---------------------------------------------------------------------------
 
-struct Geom {
-  int na, nb, nc;
-  long long plane;
-  int bx, bpr, nblocks;
-};
-
-int storage_size(int s) {
-  switch (s) {
-    case LBM_F32: return 4;
-    case LBM_F64: return 8;
-    case LBM_F16_DEV: return 2;
-    case LBM_BF16_DEV: return 2;
-    default: return 0;
-  }
-}
-
-int check_desc(const lbm_desc* d, Geom* g) {
-  if (!d) return fail(LBM_ERR_INVALID, "lbm_desc is NULL");
-  if (d->q != 9 && d->q != 19)
-    return fail(LBM_ERR_INVALID, "q must be 9 (D2Q9) or 19 (D3Q19), got " + std::to_string(d->q));
-  if (storage_size(d->storage) == 0)
-    return fail(LBM_ERR_INVALID, "unknown storage type " + std::to_string(d->storage));
-  if (d->bb_mode < 0 || d->bb_mode > 2)
-    return fail(LBM_ERR_INVALID, "unknown bb_mode " + std::to_string(d->bb_mode));
-  if (d->scheme < 0 || d->scheme > 2)
-    return fail(LBM_ERR_INVALID, "unknown scheme " + std::to_string(d->scheme));
-  if (!(d->tau > 0.5))
-    return fail(LBM_ERR_INVALID, "tau must be > 0.5 (got " + std::to_string(d->tau) + ")");
-  long long n0 = d->dims[0], n1 = d->dims[1], n2 = d->q == 19 ? d->dims[2] : 1;
-  if (n0 < 1 || n1 < 1 || n2 < 1) return fail(LBM_ERR_INVALID, "grid dims must be >= 1");
-  if (d->q == 19) {
-    g->na = (int)n0; g->nb = (int)n1; g->nc = (int)n2;
-  } else {
-    g->na = 1; g->nb = (int)n0; g->nc = (int)n1;
-  }
-  const long long plane = (long long)g->na * g->nb * g->nc;
-  if (plane >= (1LL << 31) || n0 >= (1LL << 31) || n1 >= (1LL << 31) || n2 >= (1LL << 31))
-    return fail(LBM_ERR_UNSUPPORTED, "more than 2^31-1 cells per population plane");
-  g->plane = plane;
-  int bx = d->block_x;
-  if (bx == 0) {
-    bx = g->nc >= 256 ? 256 : ((g->nc + 63) / 64) * 64;
-  }
-  if (bx < 64 || bx > 256 || bx % 64 != 0)
-    return fail(LBM_ERR_INVALID, "block_x must be 64, 128, 192 or 256");
-  g->bx = bx;
-  g->bpr = (g->nc + bx - 1) / bx;
-  const long long nblocks = (long long)g->na * g->nb * g->bpr;
-  if (nblocks >= (1LL << 31)) return fail(LBM_ERR_UNSUPPORTED, "grid too large");
-  g->nblocks = (int)nblocks;
-  return LBM_OK;
-}
-
 template <class C>
 FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
   FwdArgs<C> a;
@@ -408,7 +490,7 @@ FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
   a.na = g.na; a.nb = g.nb; a.nc = g.nc;
   a.bpr = g.bpr;
   a.nblocks = g.nblocks;
-  a.remap = (d->flags & LBM_FLAG_NO_XCD_REMAP) ? 0 : 1;
+  a.remap = (d->flags & LBM_FLAG_XCD_REMAP) ? 1 : 0;
   a.inv_tau = C(1.0 / d->tau);
   if (d->q == 19) {
     a.Fa = C(d->force[0]); a.Fb = C(d->force[1]); a.Fc = C(d->force[2]);
@@ -420,15 +502,35 @@ FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
   return a;
 }
 
-int check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess)
-    return fail(LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-  return LBM_OK;
+
+
+// Vector width of the row-per-wave kernel for this lattice (0: not eligible).
+// On MI355X the one-cell-per-thread kernel measured faster than the
+// row-per-wave kernel at every width (256^3 D3Q19: 5.6 vs 5.0 TB/s f32,
+// 5.4 vs 4.0 TB/s f16), so auto (row_vec 0) keeps it off.
+inline int row_width(const lbm_desc* d, const Geom& g) {
+  if (d->bb_mode != LBM_BB_NONE || d->row_vec <= 0) return 0;
+  return g.nc % (64 * d->row_vec) == 0 ? d->row_vec : 0;
+}
+
+template <class L, class T, int V, bool PULL, bool COLLIDE>
+int launch_row(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>::C>& a) {
+  const long long waves = (long long)g.na * g.nb * (g.nc / (64 * V));
+  const long long blocks = (waves + 3) / 4;
+  if (blocks >= (1LL << 31)) return fail(LBM_ERR_UNSUPPORTED, "grid too large");
+  hipLaunchKernelGGL((k_row<L, T, V, PULL, COLLIDE>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  return check_launch("k_row launch");
 }
 
 template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
 int launch_cell(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>::C>& a) {
+  if constexpr (BB == BB_NONE && STORE) {
+    if constexpr (sizeof(T) <= 4) {
+      if (g.rowv == 4) return launch_row<L, T, 4, PULL, COLLIDE>(s, g, a);
+    }
+    if (g.rowv == 2) return launch_row<L, T, 2, PULL, COLLIDE>(s, g, a);
+    if (g.rowv == 1) return launch_row<L, T, 1, PULL, COLLIDE>(s, g, a);
+  }
   hipLaunchKernelGGL((k_cell<L, T, BB, PULL, COLLIDE, STORE>), dim3(g.nblocks), dim3(g.bx), 0, s, a);
   return check_launch("k_cell launch");
 }
@@ -500,9 +602,11 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
   a.mask = d->bb_mode == LBM_BB_NONE ? nullptr : mask;
   a.force_field = ff;
   a.tau_field = tf;
+  Geom gr = g;
+  gr.rowv = scheme == LBM_SCHEME_PULL ? row_width(d, g) : 0;
   if (scheme == LBM_SCHEME_PULL) {
     switch (d->bb_mode) {
-      case LBM_BB_NONE: return step_pull<L, T, BB_NONE>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+      case LBM_BB_NONE: return step_pull<L, T, BB_NONE>(s, gr, a, f_in, f_out, work, nsteps, rho, u, p);
       case LBM_BB_REFERENCE: return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
       default: return step_pull<L, T, BB_HALF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
     }

@@ -49,13 +49,16 @@ class LBMKernel:
     bb : "none" (periodic), "reference" (lbm.py:944-948) or "halfway".
     lid_velocity : wall velocity of mask==2 cells under "halfway".
     scheme : "auto", "pull" or "push".
+    block_x, row_vec, xcd_remap : launch shape (see include/lbm_c_api.h);
+        0 / 0 / False pick the measured defaults.
     """
 
     def __init__(self, grid_shape: Sequence[int], lattice: str = "D3Q19",
                  tau: Optional[float] = None, viscosity: Optional[float] = None,
                  storage: str = "f32", bb: str = "none",
                  lid_velocity: Sequence[float] = (0.0, 0.0, 0.0),
-                 scheme: str = "auto", block_x: int = 0, xcd_remap: bool = True,
+                 scheme: str = "auto", block_x: int = 0, row_vec: int = 0,
+                 xcd_remap: bool = False,
                  device="cuda"):
         if lattice.upper() == "D3Q19":
             self.lat: LatticeDescriptor = d3q19()
@@ -88,6 +91,7 @@ class LBMKernel:
         self.scheme = scheme
         self.block_x = int(block_x)
         self.xcd_remap = bool(xcd_remap)
+        self.row_vec = int(row_vec)
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -115,7 +119,8 @@ class LBMKernel:
         for i in range(3):
             d.lid_velocity[i] = self.lid_velocity[i]
         d.block_x = self.block_x
-        d.flags = 0 if self.xcd_remap else _lib.LBM_FLAG_NO_XCD_REMAP
+        d.row_vec = self.row_vec
+        d.flags = _lib.LBM_FLAG_XCD_REMAP if self.xcd_remap else 0
         return d
 
     @property

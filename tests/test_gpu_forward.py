@@ -30,7 +30,7 @@ def rel(a, b):
 
 def run_both(shape, lattice, tau, nsteps, *, rho0=None, u0=None, storage="f32",
              scheme="pull", bb="none", mask=None, force=None, tau_field=None,
-             lid=(0.0, 0.0, 0.0), f0_aos=None, steps_per_call=None):
+             lid=(0.0, 0.0, 0.0), f0_aos=None, steps_per_call=None, row_vec=0):
     """Same initial state through the oracle and the HIP kernel."""
     Q = 19 if lattice == "D3Q19" else 9
     e, w, _ = orc.lattice(Q)
@@ -38,7 +38,8 @@ def run_both(shape, lattice, tau, nsteps, *, rho0=None, u0=None, storage="f32",
         rho = np.ones(shape) if rho0 is None else rho0
         u = np.zeros(tuple(shape) + (len(shape),)) if u0 is None else u0
         f0_aos = orc.equilibrium(rho, u, e, w)
-    k = kernel(shape, lattice, tau, storage=storage, scheme=scheme, bb=bb, lid_velocity=lid)
+    k = kernel(shape, lattice, tau, storage=storage, scheme=scheme, bb=bb, lid_velocity=lid,
+               row_vec=row_vec)
     f = k.from_aos(torch.from_numpy(np.ascontiguousarray(f0_aos)).to("cuda", torch.float64))
     mask_t = None if mask is None else torch.from_numpy(mask).cuda()
     force_arg = force
@@ -183,3 +184,47 @@ def test_errors_are_reported():
         k.step(f, 1)
     with pytest.raises(LBMError, match="distinct"):
         k.step(f, 1, mask=torch.zeros(8, 8, 8, dtype=torch.uint8), out=f)
+
+
+# --- row-per-wave kernels (periodic, contiguous axis a multiple of 64) -------
+ROW_CASES = [
+    ("D3Q19", (6, 5, 256), "f32", 4),     # 4 cells per lane, one wave per row
+    ("D3Q19", (4, 3, 512), "f32", 4),     # two waves per row: cross-wave edge loads
+    ("D3Q19", (5, 4, 192), "f32", 1),     # 1 cell per lane, three waves per row
+    ("D3Q19", (4, 4, 256), "f64", 2),     # 2 cells per lane, two waves per row
+    ("D3Q19", (4, 4, 128), "f16dev", 2),
+    ("D2Q9", (12, 256), "f32", 4),
+    ("D2Q9", (7, 640), "bf16dev", 2),     # five waves per row
+]
+
+
+@pytest.mark.parametrize("lattice,shape,storage,vec", ROW_CASES)
+def test_row_kernel_matches_cell_kernel_bitwise(lattice, shape, storage, vec):
+    from maddening_amd.lbm import LBMKernel
+    rng = np.random.default_rng(11)
+    D = len(shape)
+    rho = torch.from_numpy(1 + 0.01 * rng.standard_normal(shape)).float()
+    u = torch.from_numpy(0.03 * rng.standard_normal(shape + (D,))).float()
+    outs = []
+    for row_vec in (vec, -1):
+        k = LBMKernel(shape, lattice=lattice, tau=0.7, storage=storage, scheme="pull",
+                      row_vec=row_vec)
+        f = k.equilibrium(rho, u)
+        f, r, v, p = k.step(f, 9, force=[1e-5] * D)
+        torch.cuda.synchronize()
+        outs.append((f.cpu(), r.cpu(), v.cpu()))
+    for a, b in zip(outs[0], outs[1]):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("lattice,shape,storage,vec",
+                         [c for c in ROW_CASES if c[2] in ("f32", "f64")])
+def test_row_kernel_matches_oracle(lattice, shape, storage, vec):
+    rng = np.random.default_rng(12)
+    D = len(shape)
+    rho0 = 1 + 0.01 * rng.standard_normal(shape)
+    u0 = 0.03 * rng.standard_normal(shape + (D,))
+    g, o = run_both(shape, lattice, 0.7, 9, rho0=rho0, u0=u0, storage=storage,
+                    scheme="pull", force=tuple([1e-5] * D), row_vec=vec)
+    assert rel(g["f"], o["f"]) < (1e-12 if storage == "f64" else 1e-5)
+    assert rel(g["velocity"], o["velocity"]) < 1e-5
