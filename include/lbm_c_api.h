@@ -153,6 +153,50 @@ size_t lbm_reduce_workspace_size(const lbm_desc* d);
 int lbm_conservation(void* stream, const lbm_desc* d, const void* f,
                      void* work, double* out);
 
+/* ---- adjoint (reverse mode) -------------------------------------------- */
+/*
+ * No counterpart in the reference, which differentiates lbm.py:899-986 with
+ * jax.grad through lax.scan (tests/verification/test_gradient_health.py:56-75
+ * shows the pattern). These entry points are the hand-written transpose of
+ * one LBMNode.update, for f32 or f64 storage. Cotangent buffers (fbar_*) are
+ * SoA like the state and hold plain values in float (f32 storage) or double
+ * (f64 storage).
+ *
+ * Per-workgroup partial sums of dL/dF for a uniform force live in `work`
+ * (lbm_adjoint_workspace_size bytes): lbm_adjoint_begin zeroes them, every
+ * lbm_adjoint_step / lbm_moments_adjoint adds to them, lbm_adjoint_end folds
+ * them into force_bar_out (device double[3], reference axis order).
+ */
+size_t lbm_adjoint_workspace_size(const lbm_desc* d);
+int lbm_adjoint_begin(void* stream, const lbm_desc* d, void* work);
+
+/*
+ * fbar_out = (dStep/df at f_n)^T fbar_next for one step that started from
+ * f_n. Adds dL/dtau(x) into tau_bar (*G) and dL/dF(x) into force_bar_field
+ * (*G, D) when those are non-NULL; adds uniform-force partials into work
+ * when work is non-NULL. The other arguments mean what they mean in lbm_step.
+ */
+int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n,
+                     const void* fbar_next, void* fbar_out, const uint8_t* mask,
+                     const float* force_field, const float* tau_field,
+                     float* tau_bar, float* force_bar_field, void* work);
+
+/*
+ * Pull the cotangents of the output moments of state f (rho_bar, u_bar,
+ * p_bar; each nullable) back onto f: fbar_out = fbar_in (nullable) +
+ * d(moments)/df^T (...), following the moments of lbm.py:970-976 (F/2
+ * correction, u = 0 at mask cells, p = rho/3). dL/dF from the F/2 term goes
+ * to force_bar_field / work as in lbm_adjoint_step. fbar_in may equal fbar_out.
+ */
+int lbm_moments_adjoint(void* stream, const lbm_desc* d, const void* f,
+                        const uint8_t* mask, const float* force_field,
+                        const float* rho_bar, const float* u_bar,
+                        const float* p_bar, const void* fbar_in, void* fbar_out,
+                        float* force_bar_field, void* work);
+
+int lbm_adjoint_end(void* stream, const lbm_desc* d, const void* work,
+                    double* force_bar_out);
+
 #ifdef __cplusplus
 }
 #endif
