@@ -105,7 +105,11 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--scheme", default="auto")
     ap.add_argument("--block-x", type=int, default=0)
-    ap.add_argument("--no-xcd-remap", action="store_true")
+    ap.add_argument("--row-vec", type=int, default=0)
+    ap.add_argument("--xcd-remap", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB elementwise copy (torch.mul by 1) so HBM "
+                         "byte counters can be scaled by a kernel of known traffic")
     ap.add_argument("--variants", default="f16dev,bf16dev",
                     help="extra storage types to time (comma list, '' for none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -128,8 +132,16 @@ def main():
 
     def make(storage):
         return LBMKernel((n, n, n), lattice="D3Q19", tau=0.8, storage=storage,
-                         scheme=args.scheme, block_x=args.block_x,
-                         xcd_remap=not args.no_xcd_remap, device=dev)
+                         scheme=args.scheme, block_x=args.block_x, row_vec=args.row_vec,
+                         xcd_remap=args.xcd_remap, device=dev)
+
+    if args.calibrate_copy:
+        a = torch.ones(1 << 28, device=dev)  # 1 GiB float32
+        b = torch.empty_like(a)
+        for _ in range(5):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
 
     kern = make("f32")
     f0 = kern.equilibrium(rho, u)
