@@ -1,7 +1,8 @@
 """Throughput of the D3Q19 lattice-Boltzmann step on one MI355X.
 
 Workload: D3Q19 BGK, periodic 256^3 Taylor-Green vortex (u0 = 0.05,
-tau = 0.8), float32 populations. One "step" is one full lattice update
+tau = 0.8), float32 populations stored as f - w (same bytes as plain f32,
+whose rate is reported beside it). One "step" is one full lattice update
 (collision + streaming over all 16,777,216 cells). The timed region is ONE
 call of lbm_step(nsteps=K) with the state already resident in HBM, moments
 of the final state included.
@@ -86,6 +87,182 @@ def cpu_baseline(target_seconds=15.0):
                       f"(AoS, roll streaming), {steps} steps in {dt:.1f} s"}
 
 
+def precision_leg(n, dev, checkpoints=(100, 1000, 5000)):
+    """Deviation storage vs the float64 kernel on the 256^3 Taylor-Green case:
+    max |du| / u0 and max |drho| at each checkpoint, plus the float64 sums
+    of mass and momentum (drift of total mass relative to the start)."""
+    import torch
+    from maddening_amd.lbm import LBMKernel
+    u0 = 0.05
+    rho, u = taylor_green_device(n, u0, dev)
+    ref = {}
+    out = {}
+    for storage in ("f64", "f32", "f32dev", "f16dev", "bf16dev"):
+        k = LBMKernel((n, n, n), tau=0.8, storage=storage, device=dev)
+        f = k.equilibrium(rho, u)
+        m0 = k.conservation(f)
+        work = k.workspace(max(checkpoints))
+        done = 0
+        rows = {}
+        for c in checkpoints:
+            f, r, v, _ = k.step(f, c - done, work=work)
+            done = c
+            m = k.conservation(f)
+            if storage == "f64":
+                ref[c] = (r, v)
+                rows[c] = {"mass_drift": float((m[0] - m0[0]) / m0[0])}
+            else:
+                rr, vr = ref[c]
+                rows[c] = {"max_du_over_u0": float((v - vr).abs().max() / u0),
+                           "max_drho": float((r - rr).abs().max()),
+                           "rel_du": float((v - vr).abs().max() / vr.abs().max()),
+                           "mass_drift": float((m[0] - m0[0]) / m0[0])}
+        out[storage] = rows
+        del f, work, k
+        torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=4):
+    """Sphere-packed channel (porosity 0.7, radii 6-12, seed 0), tau field 0.7,
+    body force (0, 0, 1e-5), loss = mean u_z over the outlet plane. Times the
+    float32 forward and the sqrt(N)-checkpointed adjoint; checks the float64
+    adjoint against central differences of the float64 forward."""
+    import torch
+    from maddening_amd import workloads
+    from maddening_amd.adjoint import rollout
+    from maddening_amd.lbm import LBMKernel
+    mask = torch.from_numpy(workloads.sphere_packing(shape, porosity=0.7, rmin=6, rmax=12,
+                                                     seed=0)).to(dev)
+    F0 = 1e-5
+    res = {"workload": f"D3Q19 {shape} sphere-packed channel, porosity "
+                       f"{float((mask == 0).float().mean()):.3f}, tau field 0.7, F=(0,0,{F0}), "
+                       f"{nsteps} steps, loss = mean u_z over the outlet plane"}
+
+    def uz_loss(k, f, Fz):
+        e = torch.as_tensor(k.lat.e, dtype=torch.float64, device=dev)
+        fl = f[..., -1].double()
+        rh = fl.sum(0)
+        uz = ((fl * e[:, 2].view(-1, 1, 1)).sum(0) + 0.5 * Fz) / rh
+        return torch.where(mask[..., -1] != 0, torch.zeros_like(uz), uz).mean()
+
+    for storage in ("f32", "f64"):
+        k = LBMKernel(shape, tau=0.7, storage=storage, bb="reference", device=dev)
+        f0 = k.equilibrium()
+        tau = torch.full(shape, 0.7, device=dev)
+        # forward alone
+        work = k.workspace(nsteps)
+        k.step(f0, 10, mask=mask, force=[0, 0, F0], tau_field=tau, work=work)
+        torch.cuda.synchronize()
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        k.step(f0, nsteps, mask=mask, force=[0, 0, F0], tau_field=tau, work=work)
+        e1.record()
+        torch.cuda.synchronize()
+        fwd_ms = e0.elapsed_time(e1) / nsteps
+        # forward + adjoint through the autograd rollout
+        Fg = torch.tensor([0.0, 0.0, F0], dtype=torch.float64, device=dev, requires_grad=True)
+        tg = tau.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        e0.record()
+        fN, _, _, _ = rollout(k, f0, nsteps, force=Fg, tau_field=tg, mask=mask)
+        e1.record()
+        L = uz_loss(k, fN, Fg[2])
+        L.backward()
+        e2.record()
+        torch.cuda.synchronize()
+        rec = {"forward_ms_per_step": round(fwd_ms, 4),
+               "rollout_forward_ms_per_step": round(e0.elapsed_time(e1) / nsteps, 4),
+               "adjoint_ms_per_step": round(e1.elapsed_time(e2) / nsteps, 4),
+               "adjoint_over_forward": round(e1.elapsed_time(e2) / nsteps / fwd_ms, 3),
+               "fwd_plus_adjoint_ms_per_step": round(e0.elapsed_time(e2) / nsteps, 4),
+               "loss": float(L), "dL_dF": float(Fg.grad[2])}
+        if storage == "f64":
+            def loss_at(Fz, tf):
+                f, _, _, _ = k.step(f0, nsteps, mask=mask, force=[0, 0, Fz], tau_field=tf,
+                                    moments=False, work=work)
+                return float(uz_loss(k, f, Fz))
+            h = 1e-4 * F0
+            fd = (loss_at(F0 + h, tau) - loss_at(F0 - h, tau)) / (2 * h)
+            rec["fd_dL_dF"] = fd
+            rec["rel_err_dL_dF"] = abs(float(Fg.grad[2]) - fd) / abs(fd)
+            errs = []
+            fluid = (mask == 0).nonzero()
+            g = torch.Generator().manual_seed(0)
+            pick = torch.randperm(len(fluid), generator=g)[:fd_cells]
+            gmax = float(tg.grad.abs().max())
+            for i in pick.tolist():
+                c = tuple(int(v) for v in fluid[i])
+                tp, tm = tau.clone(), tau.clone()
+                tp[c] += 1e-4
+                tm[c] -= 1e-4
+                hs = float(tp[c]) - float(tm[c])
+                fdt = (loss_at(F0, tp) - loss_at(F0, tm)) / hs
+                errs.append(abs(float(tg.grad[c]) - fdt) / max(abs(fdt), 1e-3 * gmax))
+            rec["max_rel_err_dL_dtau"] = max(errs)
+            rec["fd_cells"] = fd_cells
+        res[storage] = rec
+        del k, f0, work, fN
+        torch.cuda.empty_cache()
+    return res
+
+
+def batch_leg(dev, nsteps=200, seed=0):
+    """64 lattices (32 D2Q9 edge U[64,1024], 32 D3Q19 edge U[32,160], tau
+    U[0.55,1.5], Taylor-Green or lid-driven-cavity init), float32: one
+    graph-captured launch sequence vs one lbm_step call per lattice."""
+    import torch
+    from maddening_amd import workloads
+    from maddening_amd.autotune import tune_batch
+    from maddening_amd.batch import LatticeBatch
+    specs = workloads.batch_specs(seed)
+    rng = np.random.default_rng(seed)
+    for s in specs:
+        if s["init"] == "cavity":
+            s["bb"] = "halfway"
+            s["mask"] = workloads.cavity_mask(*s["shape"])
+            s["lid_velocity"] = (0.1, 0.0, 0.0)
+    batch = LatticeBatch(specs, device=dev)
+    f0s = []
+    for k, s in zip(batch.kernels, specs):
+        if s["init"] == "cavity":
+            f0s.append(k.equilibrium())
+        elif k.D == 2:
+            r, u = workloads.taylor_green_2d(*k.grid_shape, float(rng.uniform(0.01, 0.05)))
+            f0s.append(k.equilibrium(torch.from_numpy(r).float(), torch.from_numpy(u).float()))
+        else:
+            r, u = workloads.taylor_green_3d(k.grid_shape[0], float(rng.uniform(0.01, 0.05)))
+            f0s.append(k.equilibrium(torch.from_numpy(r).float(), torch.from_numpy(u).float()))
+    tune = tune_batch(batch, use_cache=False)
+    batch.set_states(f0s)
+    h = batch.graph(nsteps, tune["grid"])
+    batch.launch_graph(h)  # warm
+    torch.cuda.synchronize()
+    batch.set_states(f0s)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    batch.launch_graph(h)
+    e1.record()
+    torch.cuda.synchronize()
+    batch.set_states(f0s)
+    batch.run_per_lattice(5)  # warm
+    torch.cuda.synchronize()
+    batch.set_states(f0s)
+    e1b = torch.cuda.Event(enable_timing=True)
+    e1b.record()
+    batch.run_per_lattice(nsteps)
+    e2.record()
+    torch.cuda.synchronize()
+    t_graph = e0.elapsed_time(e1) / 1e3
+    t_loop = e1b.elapsed_time(e2) / 1e3
+    batch.close()
+    return {"lattices": len(specs), "cells": batch.cells, "steps": nsteps,
+            "graph_MLUPS": round(batch.cells * nsteps / t_graph / 1e6, 1),
+            "per_lattice_MLUPS": round(batch.cells * nsteps / t_loop / 1e6, 1),
+            "speedup": round(t_loop / t_graph, 3),
+            "tuned": {"cells_per_thread": tune["cells_per_thread"], "grid": tune["grid"]}}
+
+
 def read_traffic():
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
@@ -110,8 +287,15 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB elementwise copy (torch.mul by 1) so HBM "
                          "byte counters can be scaled by a kernel of known traffic")
-    ap.add_argument("--variants", default="f16dev,bf16dev",
+    ap.add_argument("--storage", default="f32dev",
+                    help="population storage of the headline run: f32dev (float32 f - w, "
+                         "the default: same bytes as f32, 10x smaller rounding) or f32")
+    ap.add_argument("--variants", default="f32,f16dev,bf16dev",
                     help="extra storage types to time (comma list, '' for none)")
+    ap.add_argument("--legs", default="precision,adjoint,batch",
+                    help="extra measurements: precision (storage types vs float64 at "
+                         "100/1000/5000 steps), adjoint (forward+adjoint ms/step, gradient "
+                         "vs central differences), batch (64 uneven lattices)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -143,7 +327,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    kern = make("f32")
+    kern = make(args.storage)
     f0 = kern.equilibrium(rho, u)
     ev, wall, launches, out = time_kernel(kern, f0, args.steps, args.warmup)
     mlups = cells * args.steps / ev / 1e6
@@ -164,7 +348,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"D3Q19 BGK periodic {n}^3 Taylor-Green vortex, u0=0.05, "
-                               f"tau=0.8, float32 populations",
+                               f"tau=0.8, float32 populations ({args.storage})",
                    "cells": cells, "scheme": kern.scheme, "wall_s_timed_call": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -177,13 +361,22 @@ def main():
         k2 = make(storage)
         f2 = k2.equilibrium(rho, u)
         ev2, _, launches2, _ = time_kernel(k2, f2, args.steps, args.warmup)
-        bc2 = 2 * 19 * (2 if storage.endswith("dev") else 4)
+        bc2 = 2 * 19 * (2 if storage in ("f16dev", "bf16dev") else 4)
         extra[storage] = {"MLUPS": round(cells * args.steps / ev2 / 1e6, 1),
                           "GB/s": round(bc2 * cells / (ev2 / launches2) / 1e9, 1),
                           "speedup_vs_f32": round(ev / ev2, 3)}
         del f2, k2
     if extra:
         result["variants"] = extra
+    del out, f0
+    torch.cuda.empty_cache()
+    legs = [s for s in args.legs.split(",") if s]
+    if "precision" in legs:
+        result["precision"] = precision_leg(n, dev)
+    if "adjoint" in legs:
+        result["adjoint"] = adjoint_leg(dev)
+    if "batch" in legs:
+        result["batch"] = batch_leg(dev)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(result), flush=True)

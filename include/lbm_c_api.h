@@ -14,9 +14,9 @@
  * opposite map are those of the reference's d3q19()/d2q9()
  * (src/maddening/nodes/lbm.py:90-173).
  *
- * Storage types: LBM_F32/LBM_F64 hold f itself; LBM_F16_DEV/LBM_BF16_DEV
- * hold the deviation f_q - w_q. Collision arithmetic is FP32 for F32/F16/BF16
- * storage and FP64 for F64 storage.
+ * Storage types: LBM_F32/LBM_F64 hold f itself; LBM_F32_DEV, LBM_F16_DEV and
+ * LBM_BF16_DEV hold the deviation f_q - w_q. Collision arithmetic is FP32
+ * for every storage type but F64, which computes in FP64.
  *
  * Auxiliary per-cell arrays use the reference's layouts (last grid axis
  * contiguous): mask uint8 (*G) (0 fluid, 1 wall, 2 moving wall),
@@ -49,7 +49,8 @@ typedef enum lbm_storage {
   LBM_F32 = 0,      /* float f */
   LBM_F64 = 1,      /* double f */
   LBM_F16_DEV = 2,  /* IEEE half, f - w */
-  LBM_BF16_DEV = 3  /* bfloat16, f - w */
+  LBM_BF16_DEV = 3, /* bfloat16, f - w */
+  LBM_F32_DEV = 4   /* float, f - w */
 } lbm_storage;
 
 typedef enum lbm_bb_mode {
@@ -152,6 +153,51 @@ int lbm_convert(void* stream, const lbm_desc* d, int to_soa, int aos_is_f64,
 size_t lbm_reduce_workspace_size(const lbm_desc* d);
 int lbm_conservation(void* stream, const lbm_desc* d, const void* f,
                      void* work, double* out);
+
+/* ---- batches of independent lattices ------------------------------------ */
+/*
+ * No counterpart in the reference: its only batching is jax.vmap over
+ * equal-shaped states in GraphManager.run_sweep (graph_manager.py:2807-2877).
+ * Here lattices of any size, lattice type, tau, force and wall rule advance
+ * together: one launch per step covers all of them (float32 storage).
+ */
+typedef struct lbm_batch_item {
+  int32_t q;              /* 9 or 19 */
+  int32_t bb_mode;        /* lbm_bb_mode */
+  int64_t dims[3];        /* grid shape, reference axis order (dims[2] ignored for D2Q9) */
+  double tau;
+  double force[3];        /* uniform body force */
+  double lid_velocity[3]; /* mask==2 cells under LBM_BB_HALFWAY */
+  void* f[2];             /* two float32 SoA state buffers; a run starts from f[0] */
+  const uint8_t* mask;    /* required unless bb_mode == LBM_BB_NONE */
+  float* rho;             /* nullable moment outputs written by the last launch of a run */
+  float* u;
+  float* p;
+} lbm_batch_item;
+
+/* Bytes of device memory for the plan of `nitems` lattices. */
+size_t lbm_batch_plan_size(int nitems);
+
+/*
+ * Build the plan: validates every item, cuts each lattice into tiles of
+ * 256 * cells_per_thread consecutive cells (1..16; 0 = 1) and uploads the
+ * table to `plan`. A set-up call: it waits for the upload on `stream`.
+ * *total_tiles (nullable) receives the tile count.
+ */
+int lbm_batch_plan(void* stream, const lbm_batch_item* items, int nitems,
+                   int cells_per_thread, void* plan, long long* total_tiles);
+
+/*
+ * nsteps LBMNode.update steps of every lattice in the plan, one launch of
+ * `grid` persistent workgroups per step (nsteps + 1 launches: the pull
+ * sequence). The final state of each item is in f[(nsteps + 1) % 2].
+ */
+int lbm_batch_run(void* stream, const void* plan, int nsteps, int grid);
+
+/* The same run captured once into a HIP graph; launch it as often as needed. */
+int lbm_batch_graph_create(const void* plan, int nsteps, int grid, void** graph);
+int lbm_batch_graph_launch(void* stream, void* graph);
+int lbm_batch_graph_destroy(void* graph);
 
 /* ---- adjoint (reverse mode) -------------------------------------------- */
 /*

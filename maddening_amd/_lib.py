@@ -14,13 +14,13 @@ import torch  # noqa: F401
 
 from .build import library_path
 
-LBM_F32, LBM_F64, LBM_F16_DEV, LBM_BF16_DEV = 0, 1, 2, 3
+LBM_F32, LBM_F64, LBM_F16_DEV, LBM_BF16_DEV, LBM_F32_DEV = 0, 1, 2, 3, 4
 LBM_BB_NONE, LBM_BB_REFERENCE, LBM_BB_HALFWAY = 0, 1, 2
 LBM_SCHEME_AUTO, LBM_SCHEME_PULL, LBM_SCHEME_PUSH = 0, 1, 2
 LBM_FLAG_XCD_REMAP = 1
 
 STORAGE_NAMES = {"f32": LBM_F32, "f64": LBM_F64, "f16dev": LBM_F16_DEV,
-                 "bf16dev": LBM_BF16_DEV}
+                 "bf16dev": LBM_BF16_DEV, "f32dev": LBM_F32_DEV}
 BB_NAMES = {"none": LBM_BB_NONE, "reference": LBM_BB_REFERENCE,
             "halfway": LBM_BB_HALFWAY}
 SCHEME_NAMES = {"auto": LBM_SCHEME_AUTO, "pull": LBM_SCHEME_PULL,
@@ -44,6 +44,22 @@ class LbmDesc(ctypes.Structure):
     ]
 
 
+class LbmBatchItem(ctypes.Structure):
+    _fields_ = [
+        ("q", ctypes.c_int32),
+        ("bb_mode", ctypes.c_int32),
+        ("dims", ctypes.c_int64 * 3),
+        ("tau", ctypes.c_double),
+        ("force", ctypes.c_double * 3),
+        ("lid_velocity", ctypes.c_double * 3),
+        ("f", ctypes.c_void_p * 2),
+        ("mask", ctypes.c_void_p),
+        ("rho", ctypes.c_void_p),
+        ("u", ctypes.c_void_p),
+        ("p", ctypes.c_void_p),
+    ]
+
+
 _vp = ctypes.c_void_p
 _PROTOS = {
     "lbm_version": (ctypes.c_int, []),
@@ -58,6 +74,14 @@ _PROTOS = {
     "lbm_convert": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), ctypes.c_int, ctypes.c_int,
                                    _vp, _vp]),
     "lbm_reduce_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(LbmDesc)]),
+    "lbm_batch_plan_size": (ctypes.c_size_t, [ctypes.c_int]),
+    "lbm_batch_plan": (ctypes.c_int, [_vp, ctypes.POINTER(LbmBatchItem), ctypes.c_int, ctypes.c_int,
+                                      _vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "lbm_batch_run": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "lbm_batch_graph_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+    "lbm_batch_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "lbm_batch_graph_destroy": (ctypes.c_int, [_vp]),
     "lbm_adjoint_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(LbmDesc)]),
     "lbm_adjoint_begin": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp]),
     "lbm_adjoint_step": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp, _vp, _vp, _vp,

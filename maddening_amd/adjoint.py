@@ -133,8 +133,8 @@ def rollout(kern: LBMKernel, f0: torch.Tensor, nsteps: int, *,
     ``tau_field`` is None or a (*G) tensor; either may require grad, as may
     ``f0``. Needs f32 or f64 storage.
     """
-    if kern.storage not in ("f32", "f64"):
-        raise ValueError("differentiable rollouts need f32 or f64 storage")
+    if kern.storage not in ("f32", "f32dev", "f64"):
+        raise ValueError("differentiable rollouts need f32, f32dev or f64 storage")
     if nsteps < 1:
         raise ValueError("nsteps must be >= 1")
     if force is not None and not torch.is_tensor(force):

@@ -1,0 +1,139 @@
+"""Batches of independent lattices advanced by one launch per step.
+
+``LatticeBatch`` owns two float32 SoA state buffers per lattice and the
+device plan of ``lbm_batch_plan``. ``run(nsteps)`` launches the whole batch
+(nsteps + 1 launches); ``graph(nsteps)`` captures those launches once into a
+HIP graph that ``launch_graph`` replays. ``run_per_lattice`` is the
+comparison point: one ``lbm_step`` call per lattice.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .lbm import LBMKernel, _ptr, _stream
+
+
+class LatticeBatch:
+    """Lattices of any size and type stepped together.
+
+    ``specs``: sequence of dicts with keys ``lattice`` ("D3Q19"/"D2Q9"),
+    ``shape``, ``tau``, optional ``force`` (D,), ``bb`` ("none",
+    "reference", "halfway"), ``mask`` (uint8 array), ``lid_velocity``.
+    """
+
+    def __init__(self, specs: Sequence[dict], *, cells_per_thread: int = 1, device="cuda"):
+        self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.kernels = []
+        self.buffers = []
+        self.masks = []
+        self.specs = list(specs)
+        for s in self.specs:
+            bb = s.get("bb", "none")
+            k = LBMKernel(s["shape"], lattice=s["lattice"], tau=s["tau"], storage="f32", bb=bb,
+                          lid_velocity=s.get("lid_velocity", (0.0, 0.0, 0.0)), device=self.device)
+            self.kernels.append(k)
+            self.buffers.append((k.empty(), k.empty()))
+            m = s.get("mask")
+            self.masks.append(None if m is None else
+                              torch.as_tensor(np.asarray(m, np.uint8), device=self.device))
+        self.cells = sum(k.ncells for k in self.kernels)
+        self._plan = None
+        self._graphs = []
+        self.cells_per_thread = 0
+        self.total_tiles = 0
+        self.plan(cells_per_thread)
+
+    def _items(self):
+        arr = (_lib.LbmBatchItem * len(self.kernels))()
+        for i, (k, s) in enumerate(zip(self.kernels, self.specs)):
+            it = arr[i]
+            it.q = k.Q
+            it.bb_mode = _lib.BB_NAMES[k.bb]
+            dims = list(k.grid_shape) + [1] * (3 - k.D)
+            for j in range(3):
+                it.dims[j] = dims[j]
+            it.tau = k.tau
+            for j, v in enumerate(s.get("force", (0.0,) * k.D)):
+                it.force[j] = float(v)
+            for j in range(3):
+                it.lid_velocity[j] = k.lid_velocity[j]
+            it.f[0] = self.buffers[i][0].data_ptr()
+            it.f[1] = self.buffers[i][1].data_ptr()
+            it.mask = None if self.masks[i] is None else self.masks[i].data_ptr()
+        return arr
+
+    def plan(self, cells_per_thread: int) -> None:
+        """(Re)build the device plan with tiles of 256 * cells_per_thread cells.
+        Graphs captured from the previous plan are released."""
+        self.close()
+        L = _lib.lib()
+        n = len(self.kernels)
+        plan = torch.empty(L.lbm_batch_plan_size(n), dtype=torch.uint8, device=self.device)
+        tiles = ctypes.c_longlong(0)
+        items = self._items()
+        _lib.check(L.lbm_batch_plan(_stream(self.device), items, n, int(cells_per_thread),
+                                    _ptr(plan), ctypes.byref(tiles)), "lbm_batch_plan")
+        self._plan = plan
+        self.cells_per_thread = int(cells_per_thread)
+        self.total_tiles = int(tiles.value)
+
+    def default_grid(self) -> int:
+        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+        return int(min(self.total_tiles, 8 * cus))
+
+    # -- state ---------------------------------------------------------------
+    def set_states(self, states: Sequence[torch.Tensor]) -> None:
+        """Copy initial SoA f32 states into the first buffer of each lattice."""
+        for (a, _), f in zip(self.buffers, states):
+            a.copy_(f)
+
+    def states(self, nsteps: int):
+        """Buffers holding the final state after a run of ``nsteps`` steps."""
+        return [bufs[(nsteps + 1) % 2] for bufs in self.buffers]
+
+    # -- execution -------------------------------------------------------------
+    def run(self, nsteps: int, grid: Optional[int] = None) -> None:
+        g = grid or self.default_grid()
+        _lib.check(_lib.lib().lbm_batch_run(_stream(self.device), _ptr(self._plan), int(nsteps),
+                                            int(g)), "lbm_batch_run")
+
+    def graph(self, nsteps: int, grid: Optional[int] = None) -> ctypes.c_void_p:
+        """Capture a run of ``nsteps`` into a HIP graph (freed with the batch)."""
+        g = grid or self.default_grid()
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().lbm_batch_graph_create(_ptr(self._plan), int(nsteps), int(g),
+                                                     ctypes.byref(h)), "lbm_batch_graph_create")
+        self._graphs.append(h)
+        return h
+
+    def launch_graph(self, h: ctypes.c_void_p) -> None:
+        _lib.check(_lib.lib().lbm_batch_graph_launch(_stream(self.device), h),
+                   "lbm_batch_graph_launch")
+
+    def run_per_lattice(self, nsteps: int):
+        """Baseline: one lbm_step call per lattice, from the first buffer into the
+        second (nsteps + 1 launches per lattice). Returns the second buffers."""
+        outs = []
+        for k, (a, b), m, s in zip(self.kernels, self.buffers, self.masks, self.specs):
+            k.step(a, nsteps, mask=m, force=list(s.get("force", (0.0,) * k.D)), out=b,
+                   moments=False)
+            outs.append(b)
+        return outs
+
+    def close(self) -> None:
+        for h in self._graphs:
+            _lib.lib().lbm_batch_graph_destroy(h)
+        self._graphs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

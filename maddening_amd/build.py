@@ -26,6 +26,9 @@ HIPCC_FLAGS = [
     # packed-FP32 VALU forms stay off: plain scalar FP32 code throughout
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # no implicit a*b+c fusion: every kernel variant (single lattice, row,
+    # batched) evaluates the collision with the same roundings
+    "-ffp-contract=off",
     "-Wall",
     "-Wno-unused-function",
 ]

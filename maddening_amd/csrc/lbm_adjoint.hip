@@ -283,8 +283,8 @@ __global__ void __launch_bounds__(256) k_partial_sum(const double* partial, long
 inline int adj_geom(const lbm_desc* d, Geom* g) {
   int rc = check_desc(d, g);
   if (rc) return rc;
-  if (d->storage != LBM_F32 && d->storage != LBM_F64)
-    return fail(LBM_ERR_UNSUPPORTED, "the adjoint needs f32 or f64 storage");
+  if (d->storage != LBM_F32 && d->storage != LBM_F64 && d->storage != LBM_F32_DEV)
+    return fail(LBM_ERR_UNSUPPORTED, "the adjoint needs f32, f32dev or f64 storage");
   g->bx = 256;
   g->bpr = (g->nc + 255) / 256;
   g->nblocks = g->na * g->nb * g->bpr;
@@ -403,12 +403,18 @@ int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, const voi
   if (d->bb_mode != LBM_BB_NONE && !mask) return fail(LBM_ERR_INVALID, "bb_mode needs a wall mask");
   hipStream_t s = (hipStream_t)stream;
   if (d->q == 19) {
+    if (d->storage == LBM_F32_DEV)
+      return adjoint_typed<Lattice<19>, f32dev_t>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+                                                 tau_field, tau_bar, force_bar_field, work);
     if (d->storage == LBM_F32)
       return adjoint_typed<Lattice<19>, float>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
                                               tau_field, tau_bar, force_bar_field, work);
     return adjoint_typed<Lattice<19>, double>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
                                              tau_field, tau_bar, force_bar_field, work);
   }
+  if (d->storage == LBM_F32_DEV)
+    return adjoint_typed<Lattice<9>, f32dev_t>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+                                              tau_field, tau_bar, force_bar_field, work);
   if (d->storage == LBM_F32)
     return adjoint_typed<Lattice<9>, float>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
                                            tau_field, tau_bar, force_bar_field, work);
@@ -428,12 +434,18 @@ int lbm_moments_adjoint(void* stream, const lbm_desc* d, const void* f, const ui
   if (!f || !fbar_out) return fail(LBM_ERR_INVALID, "f and fbar_out must be non-NULL");
   hipStream_t s = (hipStream_t)stream;
   if (d->q == 19) {
+    if (d->storage == LBM_F32_DEV)
+      return moments_adjoint_typed<Lattice<19>, f32dev_t>(s, d, g, f, mask, force_field, rho_bar, u_bar,
+                                                         p_bar, fbar_in, fbar_out, force_bar_field, work);
     if (d->storage == LBM_F32)
       return moments_adjoint_typed<Lattice<19>, float>(s, d, g, f, mask, force_field, rho_bar, u_bar,
                                                       p_bar, fbar_in, fbar_out, force_bar_field, work);
     return moments_adjoint_typed<Lattice<19>, double>(s, d, g, f, mask, force_field, rho_bar, u_bar,
                                                      p_bar, fbar_in, fbar_out, force_bar_field, work);
   }
+  if (d->storage == LBM_F32_DEV)
+    return moments_adjoint_typed<Lattice<9>, f32dev_t>(s, d, g, f, mask, force_field, rho_bar, u_bar,
+                                                      p_bar, fbar_in, fbar_out, force_bar_field, work);
   if (d->storage == LBM_F32)
     return moments_adjoint_typed<Lattice<9>, float>(s, d, g, f, mask, force_field, rho_bar, u_bar,
                                                    p_bar, fbar_in, fbar_out, force_bar_field, work);

@@ -1,0 +1,248 @@
+// Many independent lattices (D2Q9 and D3Q19, any sizes, own tau / force /
+// walls) advanced by ONE launch per step.
+//
+// The plan is a table in device memory: one entry per lattice and a running
+// tile count. A tile is 256 * cells_per_thread consecutive cells of one
+// lattice (row-major over the whole grid, so short rows leave no thread
+// idle); every lattice is cut into equal-sized pieces whatever its shape.
+// A persistent grid of workgroups strides over all tiles of all lattices; a
+// workgroup finds its lattice by binary search over the tile offsets and runs
+// the same per-cell update as the single-lattice kernels (lbm_cell.hpp).
+// Steps follow the pull sequence: launch k reads f[k % 2] and writes
+// f[(k + 1) % 2]; launch 0 collides, launches 1 .. n-1 stream and collide,
+// launch n streams (and writes moments where an entry asks for them).
+// A whole n-step run can be captured once into a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm_c_api.h"
+#include "lbm_cell.hpp"
+#include "lbm_common.hpp"
+#include "lbm_host.hpp"
+
+namespace mlbm {
+
+struct BatchEntry {
+  int q, bb;
+  int na, nb, nc, bpr;
+  long long plane;
+  long long tile_begin;
+  float inv_tau, Fa, Fb, Fc, lid_a, lid_b, lid_c;
+  int pad;
+  void* buf[2];
+  const uint8_t* mask;
+  float* rho;
+  float* u;
+  float* p;
+};
+
+struct BatchHeader {
+  long long total_tiles;
+  int nitems;
+  int cells_per_thread;
+};
+
+constexpr size_t kHeaderBytes = 64;  // keeps the entry table 64-byte aligned
+
+__host__ __device__ inline const BatchEntry* entries(const void* plan) {
+  return reinterpret_cast<const BatchEntry*>(static_cast<const char*>(plan) + kHeaderBytes);
+}
+
+template <class L, bool PULL, bool COLLIDE>
+__device__ __forceinline__ void batch_cell(const FwdArgs<float>& a, int bb, int row, int ic) {
+  if (bb == BB_NONE) cell_update<L, float, BB_NONE, PULL, COLLIDE, true>(a, row, ic);
+  else if (bb == BB_REF) cell_update<L, float, BB_REF, PULL, COLLIDE, true>(a, row, ic);
+  else cell_update<L, float, BB_HALF, PULL, COLLIDE, true>(a, row, ic);
+}
+
+// PHASE 0: collide only; 1: stream + collide; 2: stream (+ moments)
+template <int PHASE>
+__global__ void __launch_bounds__(256) k_batch(const void* plan, int parity) {
+  const BatchHeader* h = static_cast<const BatchHeader*>(plan);
+  const BatchEntry* items = entries(plan);
+  const long long total = h->total_tiles;
+  const int n = h->nitems;
+  const int cpt = h->cells_per_thread;
+  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].tile_begin <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchEntry& e = items[lo];
+    const long long base = (t - e.tile_begin) * (256LL * cpt) + threadIdx.x;
+    FwdArgs<float> a = {};
+    a.src = e.buf[parity];
+    a.dst = e.buf[parity ^ 1];
+    a.mask = e.mask;
+    a.plane = e.plane;
+    a.na = e.na; a.nb = e.nb; a.nc = e.nc;
+    a.inv_tau = e.inv_tau;
+    a.Fa = e.Fa; a.Fb = e.Fb; a.Fc = e.Fc;
+    a.lid_a = e.lid_a; a.lid_b = e.lid_b; a.lid_c = e.lid_c;
+    if (PHASE == 2) {
+      a.rho = e.rho; a.u = e.u; a.p = e.p;
+    }
+    constexpr bool PULL = PHASE != 0;
+    constexpr bool COLLIDE = PHASE != 2;
+    for (int k = 0; k < cpt; ++k) {
+      const long long cell = base + 256LL * k;
+      if (cell >= e.plane) break;
+      const int row = (int)(cell / e.nc);
+      const int ic = (int)(cell - (long long)row * e.nc);
+      if (e.q == 19) batch_cell<Lattice<19>, PULL, COLLIDE>(a, e.bb, row, ic);
+      else batch_cell<Lattice<9>, PULL, COLLIDE>(a, e.bb, row, ic);
+    }
+  }
+}
+
+inline int batch_launch(hipStream_t s, const void* plan, int phase, int parity, int grid) {
+  switch (phase) {
+    case 0: hipLaunchKernelGGL(k_batch<0>, dim3(grid), dim3(256), 0, s, plan, parity); break;
+    case 1: hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(256), 0, s, plan, parity); break;
+    default: hipLaunchKernelGGL(k_batch<2>, dim3(grid), dim3(256), 0, s, plan, parity); break;
+  }
+  return check_launch("k_batch launch");
+}
+
+inline int batch_run(hipStream_t s, const void* plan, int nsteps, int grid) {
+  int rc = batch_launch(s, plan, 0, 0, grid);
+  for (int k = 1; k < nsteps && rc == LBM_OK; ++k) rc = batch_launch(s, plan, 1, k & 1, grid);
+  if (rc == LBM_OK) rc = batch_launch(s, plan, 2, nsteps & 1, grid);
+  return rc;
+}
+
+struct GraphHandle {
+  hipGraphExec_t exec;
+};
+
+}  // namespace mlbm
+
+using namespace mlbm;
+
+extern "C" {
+
+size_t lbm_batch_plan_size(int nitems) {
+  if (nitems < 1) return 0;
+  return kHeaderBytes + sizeof(BatchEntry) * (size_t)nitems;
+}
+
+int lbm_batch_plan(void* stream, const lbm_batch_item* items, int nitems,
+                   int cells_per_thread, void* plan, long long* total_tiles) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  if (!items || nitems < 1 || !plan) return fail(LBM_ERR_INVALID, "items, nitems >= 1 and plan needed");
+  const int cpt = cells_per_thread == 0 ? 1 : cells_per_thread;
+  if (cpt < 1 || cpt > 16) return fail(LBM_ERR_INVALID, "cells_per_thread must be 1..16");
+  const long long tile_cells = 256LL * cpt;
+  std::vector<char> host(lbm_batch_plan_size(nitems), 0);
+  BatchEntry* out = reinterpret_cast<BatchEntry*>(host.data() + kHeaderBytes);
+  long long tiles = 0;
+  for (int i = 0; i < nitems; ++i) {
+    const lbm_batch_item& it = items[i];
+    lbm_desc d = {};
+    d.q = it.q;
+    d.storage = LBM_F32;
+    d.bb_mode = it.bb_mode;
+    d.tau = it.tau;
+    for (int k = 0; k < 3; ++k) d.dims[k] = it.dims[k];
+    Geom g;
+    int rc = check_desc(&d, &g);
+    if (rc) return fail(rc, "batch item " + std::to_string(i) + ": " + g_last_error);
+    if (!it.f[0] || !it.f[1] || it.f[0] == it.f[1])
+      return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": needs two distinct state buffers");
+    if (it.bb_mode != LBM_BB_NONE && !it.mask)
+      return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": bb_mode needs a mask");
+    BatchEntry& e = out[i];
+    e.q = it.q;
+    e.bb = it.bb_mode;
+    e.na = g.na; e.nb = g.nb; e.nc = g.nc;
+    e.bpr = g.bpr;
+    e.plane = g.plane;
+    e.tile_begin = tiles;
+    e.inv_tau = float(1.0 / it.tau);
+    if (it.q == 19) {
+      e.Fa = float(it.force[0]); e.Fb = float(it.force[1]); e.Fc = float(it.force[2]);
+      e.lid_a = float(it.lid_velocity[0]); e.lid_b = float(it.lid_velocity[1]);
+      e.lid_c = float(it.lid_velocity[2]);
+    } else {
+      e.Fa = 0.f; e.Fb = float(it.force[0]); e.Fc = float(it.force[1]);
+      e.lid_a = 0.f; e.lid_b = float(it.lid_velocity[0]); e.lid_c = float(it.lid_velocity[1]);
+    }
+    e.buf[0] = it.f[0];
+    e.buf[1] = it.f[1];
+    e.mask = it.bb_mode == LBM_BB_NONE ? nullptr : it.mask;
+    e.rho = it.rho;
+    e.u = it.u;
+    e.p = it.p;
+    tiles += (g.plane + tile_cells - 1) / tile_cells;
+  }
+  BatchHeader* hdr = reinterpret_cast<BatchHeader*>(host.data());
+  hdr->total_tiles = tiles;
+  hdr->nitems = nitems;
+  hdr->cells_per_thread = cpt;
+  if (total_tiles) *total_tiles = tiles;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t err = hipMemcpyAsync(plan, host.data(), host.size(), hipMemcpyHostToDevice, s);
+  if (err == hipSuccess) err = hipStreamSynchronize(s);  // the host table dies with this call
+  if (err != hipSuccess) return fail(LBM_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(err));
+  return LBM_OK;
+}
+
+int lbm_batch_run(void* stream, const void* plan, int nsteps, int grid) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  if (!plan || nsteps < 1 || grid < 1) return fail(LBM_ERR_INVALID, "plan, nsteps >= 1 and grid >= 1 needed");
+  return batch_run((hipStream_t)stream, plan, nsteps, grid);
+}
+
+int lbm_batch_graph_create(const void* plan, int nsteps, int grid, void** graph) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  if (!plan || nsteps < 1 || grid < 1 || !graph)
+    return fail(LBM_ERR_INVALID, "plan, nsteps >= 1, grid >= 1 and graph needed");
+  hipStream_t s;
+  hipError_t err = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (err != hipSuccess) return fail(LBM_ERR_HIP, std::string("stream: ") + hipGetErrorString(err));
+  hipGraph_t g = nullptr;
+  err = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  int rc = LBM_OK;
+  if (err == hipSuccess) {
+    rc = batch_run(s, plan, nsteps, grid);
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    if (err == hipSuccess) err = e2;
+  }
+  hipGraphExec_t exec = nullptr;
+  if (err == hipSuccess && rc == LBM_OK) err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(s);
+  if (rc != LBM_OK) return rc;
+  if (err != hipSuccess) return fail(LBM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(err));
+  GraphHandle* hnd = new GraphHandle{exec};
+  *graph = hnd;
+  return LBM_OK;
+}
+
+int lbm_batch_graph_launch(void* stream, void* graph) {
+  g_last_error.clear();
+  if (!graph) return fail(LBM_ERR_INVALID, "graph is NULL");
+  hipError_t err = hipGraphLaunch(static_cast<GraphHandle*>(graph)->exec, (hipStream_t)stream);
+  if (err != hipSuccess) return fail(LBM_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(err));
+  return LBM_OK;
+}
+
+int lbm_batch_graph_destroy(void* graph) {
+  g_last_error.clear();
+  if (!graph) return LBM_OK;
+  GraphHandle* h = static_cast<GraphHandle*>(graph);
+  hipError_t err = hipGraphExecDestroy(h->exec);
+  delete h;
+  if (err != hipSuccess) return fail(LBM_ERR_HIP, std::string("graph destroy: ") + hipGetErrorString(err));
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -75,6 +75,24 @@ template <> struct Storage<double> {
   }
 };
 
+// float32 deviation f - w: same 4 bytes as f32, but the rounding of the
+// stored value scales with |f - w| instead of with w.
+struct f32dev_t {
+  float v;
+};
+
+template <> struct Storage<f32dev_t> {
+  using C = float;
+  template <class L>
+  static __device__ __forceinline__ float load(const f32dev_t* p, int) {
+    return p->v;
+  }
+  template <class L>
+  static __device__ __forceinline__ void store(f32dev_t* p, int, float d) {
+    __builtin_nontemporal_store(d, &p->v);
+  }
+};
+
 template <> struct Storage<_Float16> {
   using C = float;
   template <class L>
@@ -139,6 +157,19 @@ template <class L, class C>
 __device__ __forceinline__ void collide(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
                                         C Fc, C drho, C rho, C ua, C ub,
                                         C uc) {
+  // raw first moment of the incoming populations (used by the float32 fix-up)
+  C ja = 0, jb = 0, jc = 0;
+  if constexpr (sizeof(C) == 4) {
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+      if (L::ea[q] > 0) ja += d[q];
+      if (L::ea[q] < 0) ja -= d[q];
+      if (L::eb[q] > 0) jb += d[q];
+      if (L::eb[q] < 0) jb -= d[q];
+      if (L::ec[q] > 0) jc += d[q];
+      if (L::ec[q] < 0) jc -= d[q];
+    }
+  }
   const C usq15 = C(1.5) * (ua * ua + ub * ub + uc * uc);
   const C uF = ua * Fa + ub * Fb + uc * Fc;
   const C keep = C(1) - inv_tau;
@@ -151,6 +182,33 @@ __device__ __forceinline__ void collide(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
     const C deq = w * (drho + rho * (C(3) * eu + C(4.5) * eu * eu - usq15));
     const C S = src * w * (C(3) * (eF - uF) + C(9) * eu * eF);
     d[q] = keep * d[q] + inv_tau * deq + S;
+  }
+  if constexpr (sizeof(C) == 4) {
+    // In exact arithmetic the collision keeps sum d = rho - 1 and moves
+    // sum d e from j to j + F. Float32 rounding misses both by about an ulp
+    // per step, and that error accumulates in rho and u over a run. Project
+    // the miss back onto the conserved moments: adding w_q (a + 3 e_q . b)
+    // changes sum d by a and sum d e by b, because sum w = 1, sum w e = 0
+    // and sum w e e = I/3.
+    C m0 = 0, ma = 0, mb = 0, mc = 0;
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+      m0 += d[q];
+      if (L::ea[q] > 0) ma += d[q];
+      if (L::ea[q] < 0) ma -= d[q];
+      if (L::eb[q] > 0) mb += d[q];
+      if (L::eb[q] < 0) mb -= d[q];
+      if (L::ec[q] > 0) mc += d[q];
+      if (L::ec[q] < 0) mc -= d[q];
+    }
+    // targets: rho - 1 (unchanged) and j + F
+    const C a0 = drho - m0;
+    const C ba = C(3) * ((ja + Fa) - ma);
+    const C bb = C(3) * ((jb + Fb) - mb);
+    const C bc = C(3) * ((jc + Fc) - mc);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q)
+      d[q] += weight<L, C>(q) * (a0 + C(L::ea[q]) * ba + C(L::eb[q]) * bb + C(L::ec[q]) * bc);
   }
 }
 

@@ -20,145 +20,15 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../include/lbm_c_api.h"
 #include "lbm_common.hpp"
 #include "lbm_host.hpp"
+#include "lbm_cell.hpp"
 
 namespace mlbm {
 
-
-template <class C>
-struct FwdArgs {
-  const void* src;
-  void* dst;
-  const uint8_t* mask;
-  const float* force_field;
-  const float* tau_field;
-  float* rho;
-  float* u;
-  float* p;
-  long long plane;  // cells per population plane
-  int na, nb, nc;
-  int bpr;          // workgroups per grid row
-  int nblocks;
-  int remap;
-  C inv_tau;
-  C Fa, Fb, Fc;
-  C lid_a, lid_b, lid_c;
-};
-
-template <class L, class C>
-__device__ __forceinline__ void load_force(const FwdArgs<C>& a, int cell, C& Fa,
-                                           C& Fb, C& Fc) {
-  Fa = a.Fa;
-  Fb = a.Fb;
-  Fc = a.Fc;
-  if (a.force_field) {
-    if (L::D == 3) {
-      Fa = C(a.force_field[3 * (long long)cell + 0]);
-      Fb = C(a.force_field[3 * (long long)cell + 1]);
-      Fc = C(a.force_field[3 * (long long)cell + 2]);
-    } else {
-      Fb = C(a.force_field[2 * (long long)cell + 0]);
-      Fc = C(a.force_field[2 * (long long)cell + 1]);
-    }
-  }
-}
-
-template <class L, class C>
-__device__ __forceinline__ void write_moments(const FwdArgs<C>& a, int cell,
-                                              bool wall, C rho, C ua, C ub,
-                                              C uc) {
-  if (a.rho) a.rho[cell] = float(rho);
-  if (a.p) a.p[cell] = float(rho * C(1.0 / 3.0));
-  if (a.u) {
-    if (wall) ua = ub = uc = C(0);
-    if (L::D == 3) {
-      a.u[3 * (long long)cell + 0] = float(ua);
-      a.u[3 * (long long)cell + 1] = float(ub);
-      a.u[3 * (long long)cell + 2] = float(uc);
-    } else {
-      a.u[2 * (long long)cell + 0] = float(ub);
-      a.u[2 * (long long)cell + 1] = float(uc);
-    }
-  }
-}
-
-// 2 w_q (e_q . u_wall) / cs2: momentum a moving wall adds to the population
-// it reflects into direction q (halfway bounce-back, wall density 1).
-template <class L, class C>
-__device__ __forceinline__ C lid_term(const FwdArgs<C>& a, int q) {
-  return C(6) * weight<L, C>(q) *
-         (C(L::ea[q]) * a.lid_a + C(L::eb[q]) * a.lid_b + C(L::ec[q]) * a.lid_c);
-}
-
-// PULL: gather (stream + bounce-back) instead of an aligned read.
-// COLLIDE: apply the collision after the read.
-// STORE: write the state back (false for the moments-only pass).
-template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
-__global__ void __launch_bounds__(256)
-    k_cell(FwdArgs<typename Storage<T>::C> a) {
-  using C = typename Storage<T>::C;
-  using St = Storage<T>;
-  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
-  const int row = lb / a.bpr;
-  const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
-  if (ic >= a.nc) return;
-  const int ia = row / a.nb;
-  const int ib = row - ia * a.nb;
-  Nbr n;
-  n.init(ia, ib, ic, a.na, a.nb, a.nc);
-  const T* src = static_cast<const T*>(a.src);
-  const long long P = a.plane;
-  const uint8_t self = a.mask ? a.mask[n.cell] : uint8_t(0);
-
-  C d[L::Q];
-  if (BB == BB_HALF && PULL && self) {
-    // solid cell under halfway bounce-back: no fluid reads it; park it at rest
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) d[q] = C(0);
-  } else {
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) {
-      if (!PULL) {
-        d[q] = St::template load<L>(src + q * P + n.cell, q);
-      } else if (BB == BB_REF) {
-        // reference bounce-back (lbm.py:944-948): at a wall cell slot q takes
-        // the value streamed into slot opp(q), which came from x + e_q.
-        const T* ptr = self ? src + L::opp[q] * P + n.template at<L>(q, +1)
-                            : src + q * P + n.template at<L>(q, -1);
-        d[q] = St::template load<L>(ptr, q);
-      } else if (BB == BB_HALF) {
-        const int s = n.template at<L>(q, -1);
-        const uint8_t m = (q == 0) ? uint8_t(0) : a.mask[s];
-        if (m) {
-          d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) +
-                 (m == 2 ? lid_term<L, C>(a, q) : C(0));
-        } else {
-          d[q] = St::template load<L>(src + q * P + s, q);
-        }
-      } else {
-        d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
-      }
-    }
-  }
-
-  C Fa, Fb, Fc;
-  load_force<L, C>(a, n.cell, Fa, Fb, Fc);
-  C drho, rho, ua, ub, uc;
-  moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
-  if (a.rho || a.u || a.p) write_moments<L, C>(a, n.cell, self != 0, rho, ua, ub, uc);
-  if (COLLIDE) {
-    const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
-    collide<L, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
-  }
-  if (STORE) {
-    T* dst = static_cast<T*>(a.dst);
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) St::template store<L>(dst + q * P + n.cell, q, d[q]);
-  }
-}
 
 // Push: aligned read of the reference state f(x), collision, scatter of f*(x)
 // to x + e_q. Under reference bounce-back a value entering a wall cell lands
@@ -524,7 +394,7 @@ int launch_row(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>::
 
 template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
 int launch_cell(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>::C>& a) {
-  if constexpr (BB == BB_NONE && STORE) {
+  if constexpr (BB == BB_NONE && STORE && !std::is_same<T, f32dev_t>::value) {
     if constexpr (sizeof(T) <= 4) {
       if (g.rowv == 4) return launch_row<L, T, 4, PULL, COLLIDE>(s, g, a);
     }
@@ -625,6 +495,7 @@ int dispatch(const lbm_desc* d, Args&&... args) {
       case LBM_F32: return F<Lattice<19>, float>::run(args...);
       case LBM_F64: return F<Lattice<19>, double>::run(args...);
       case LBM_F16_DEV: return F<Lattice<19>, _Float16>::run(args...);
+      case LBM_F32_DEV: return F<Lattice<19>, f32dev_t>::run(args...);
       default: return F<Lattice<19>, __bf16>::run(args...);
     }
   }
@@ -632,6 +503,7 @@ int dispatch(const lbm_desc* d, Args&&... args) {
     case LBM_F32: return F<Lattice<9>, float>::run(args...);
     case LBM_F64: return F<Lattice<9>, double>::run(args...);
     case LBM_F16_DEV: return F<Lattice<9>, _Float16>::run(args...);
+    case LBM_F32_DEV: return F<Lattice<9>, f32dev_t>::run(args...);
     default: return F<Lattice<9>, __bf16>::run(args...);
   }
 }

@@ -33,6 +33,7 @@ inline int storage_size(int s) {
     case LBM_F64: return 8;
     case LBM_F16_DEV: return 2;
     case LBM_BF16_DEV: return 2;
+    case LBM_F32_DEV: return 4;
     default: return 0;
   }
 }
@@ -70,6 +71,8 @@ inline int check_desc(const lbm_desc* d, Geom* g) {
     return fail(LBM_ERR_INVALID, "row_vec must be -1, 0, 1, 2 or 4");
   if (d->row_vec == 4 && d->storage == LBM_F64)
     return fail(LBM_ERR_UNSUPPORTED, "row_vec 4 is not built for f64 storage (use 1 or 2)");
+  if (d->row_vec > 0 && d->storage == LBM_F32_DEV)
+    return fail(LBM_ERR_UNSUPPORTED, "the row-per-wave kernels are not built for f32dev storage");
   if (d->row_vec > 0 && g->nc % (64 * d->row_vec) != 0)
     return fail(LBM_ERR_INVALID, "row_vec needs the contiguous axis to be a multiple of 64*row_vec");
   g->bx = bx;

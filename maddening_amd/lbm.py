@@ -4,7 +4,8 @@ Tensors carry device memory and streams; every computation is a call into
 libmaddening_lbm.so through its C ABI (include/lbm_c_api.h).
 
 States are SoA tensors of shape (Q, *grid_shape) in the storage dtype:
-float32 / float64 hold f itself, float16 / bfloat16 hold f - w. ``aos_view``
+"f32" / "f64" hold f itself; "f32dev", "f16dev" and "bf16dev" hold f - w
+(float32, float16, bfloat16). ``aos_view``
 gives the reference's (*grid_shape, Q) view of an f32/f64 state without a
 copy.
 """
@@ -24,6 +25,7 @@ _TORCH_DTYPE = {
     _lib.LBM_F64: torch.float64,
     _lib.LBM_F16_DEV: torch.float16,
     _lib.LBM_BF16_DEV: torch.bfloat16,
+    _lib.LBM_F32_DEV: torch.float32,
 }
 
 
@@ -45,7 +47,7 @@ class LBMKernel:
     lattice : "D3Q19" or "D2Q9".
     tau : BGK relaxation time (> 0.5). Alternatively ``viscosity`` with
         tau = 0.5 + viscosity / cs2 (lbm.py:634).
-    storage : "f32", "f64", "f16dev" or "bf16dev".
+    storage : "f32", "f64", "f32dev", "f16dev" or "bf16dev".
     bb : "none" (periodic), "reference" (lbm.py:944-948) or "halfway".
     lid_velocity : wall velocity of mask==2 cells under "halfway".
     scheme : "auto", "pull" or "push".

@@ -117,6 +117,9 @@ class LBMNode(SimulationNode):
                                  f"but lattice {lat.name} only has {lat.D} dimensions.")
         if storage not in ("f32", "f64"):
             raise ValueError("LBMNode state holds f itself: storage must be 'f32' or 'f64'")
+        if wall_mask is not None and tuple(np.shape(wall_mask)) != tuple(grid_shape):
+            raise ValueError(f"wall_mask shape {tuple(np.shape(wall_mask))} != grid_shape "
+                             f"{tuple(grid_shape)}")
         super().__init__(name, timestep, grid_shape=grid_shape, viscosity=viscosity,
                          lattice=lattice, inlet_face=inlet_face, outlet_face=outlet_face,
                          geometry_source=geometry_source)
@@ -133,9 +136,6 @@ class LBMNode(SimulationNode):
         self._device = self._kernel.device
         self._dtype = self._kernel.dtype
         if wall_mask is not None:
-            if tuple(np.shape(wall_mask)) != self._grid_shape:
-                raise ValueError(f"wall_mask shape {tuple(np.shape(wall_mask))} != grid_shape "
-                                 f"{self._grid_shape}")
             wm = np.asarray(wall_mask)
             self._has_walls = bool(np.any(wm))
             self._wall_mask = torch.as_tensor(wm.astype(np.uint8), device=self._device)

@@ -188,7 +188,7 @@ def test_float32_adjoint_close_to_float64():
     mask = torch.from_numpy(workloads.sphere_packing(shape, porosity=0.8, rmin=2, rmax=3,
                                                      seed=3)).cuda()
     grads = {}
-    for storage in ("f32", "f64"):
+    for storage in ("f32", "f32dev", "f64"):
         k = LBMKernel(shape, tau=0.7, storage=storage, bb="reference")
         Fg = torch.tensor([0.0, 0.0, 1e-5], dtype=torch.float64, device="cuda",
                           requires_grad=True)
@@ -196,5 +196,6 @@ def test_float32_adjoint_close_to_float64():
         _, _, u, _ = rollout(k, k.equilibrium(), nsteps, force=Fg, tau_field=tg, mask=mask)
         u[..., -1, 2].double().mean().backward()
         grads[storage] = (Fg.grad.clone(), tg.grad.clone())
-    assert rel(grads["f32"][0], grads["f64"][0]) < 1e-4
-    assert rel(grads["f32"][1], grads["f64"][1]) < 1e-3
+    for s in ("f32", "f32dev"):
+        assert rel(grads[s][0], grads["f64"][0]) < 1e-4
+        assert rel(grads[s][1], grads["f64"][1]) < 1e-3
