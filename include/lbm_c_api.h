@@ -122,6 +122,19 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out,
              int nsteps);
 
 /*
+ * One launch of the pull sequence lbm_step runs (its building block, for
+ * callers that keep post-collision states, like the adjoint's recompute):
+ *   phase 0: src = a state f_n, dst = its collision C(f_n);
+ *   phase 1: src = post-collision g_{n-1}, dst = C(f_n) with f_n gathered
+ *            from g_{n-1} by streaming and bounce-back;
+ *   phase 2: src = post-collision g_{n-1}, dst = f_n, and the moments of f_n
+ *            into rho/u/p (each nullable).
+ */
+int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, void* dst,
+                   const uint8_t* mask, const float* force_field, const float* tau_field,
+                   float* rho, float* u, float* p);
+
+/*
  * Output moments of a state without stepping: _compute_macroscopic with the
  * Guo F/2 half-correction (lbm.py:209-239), u zeroed at mask cells and
  * p = rho * cs2 (lbm.py:971-976).
@@ -218,11 +231,15 @@ int lbm_adjoint_begin(void* stream, const lbm_desc* d, void* work);
 
 /*
  * fbar_out = (dStep/df at f_n)^T fbar_next for one step that started from
- * f_n. Adds dL/dtau(x) into tau_bar (*G) and dL/dF(x) into force_bar_field
- * (*G, D) when those are non-NULL; adds uniform-force partials into work
- * when work is non-NULL. The other arguments mean what they mean in lbm_step.
+ * f_n. With f_n_post == 0, `f_n` is that state; with f_n_post != 0 it is the
+ * post-collision state of the previous step (what lbm_pull_phase phases 0
+ * and 1 write) and f_n is gathered from it by the forward's own streaming
+ * and bounce-back. Adds dL/dtau(x) into tau_bar (*G) and dL/dF(x) into
+ * force_bar_field (*G, D) when those are non-NULL; adds uniform-force
+ * partials into work when work is non-NULL. The other arguments mean what
+ * they mean in lbm_step.
  */
-int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n,
+int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, int f_n_post,
                      const void* fbar_next, void* fbar_out, const uint8_t* mask,
                      const float* force_field, const float* tau_field,
                      float* tau_bar, float* force_bar_field, void* work);

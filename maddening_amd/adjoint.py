@@ -15,7 +15,6 @@ force field (*G, D), and to a per-cell relaxation-time field (*G).
 """
 from __future__ import annotations
 
-import copy
 import ctypes
 import math
 from typing import Optional, Union
@@ -41,16 +40,18 @@ class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f0, force, tau_field, kern: LBMKernel, nsteps: int, mask, seg: int):
         ctx.set_materialize_grads(False)
+        # a uniform force goes to the kernels as host numbers: read it once
+        fk = force.tolist() if (force is not None and force.dim() == 1) else force
         ckpts = []
         f = f0
         done = 0
         while done < nsteps:
             n = min(seg, nsteps - done)
             ckpts.append(f)
-            f, _, _, _ = kern.step(f, n, mask=mask, force=force, tau_field=tau_field,
+            f, _, _, _ = kern.step(f, n, mask=mask, force=fk, tau_field=tau_field,
                                    moments=False)
             done += n
-        rho, u, p = kern.moments(f, mask=mask, force=force)
+        rho, u, p = kern.moments(f, mask=mask, force=fk)
         ctx.kern = kern
         ctx.nsteps = nsteps
         ctx.seg = seg
@@ -58,6 +59,7 @@ class _Rollout(torch.autograd.Function):
         ctx.force_is_field = force is not None and force.dim() > 1
         ctx.force_shape = None if force is None else tuple(force.shape)
         ctx.force_dtype = None if force is None else force.dtype
+        ctx.force_list = fk if isinstance(fk, list) else None
         ctx.save_for_backward(f, force, tau_field, *ckpts)
         return f, rho, u, p
 
@@ -68,7 +70,10 @@ class _Rollout(torch.autograd.Function):
         L = _lib.lib()
         dev = kern.device
         st = _stream(dev)
-        ff, fu = kern._field(force)
+        if ctx.force_list is not None:
+            ff, fu = None, ctx.force_list
+        else:
+            ff, fu = kern._field(force)
         d = kern.desc(force=fu)
         m = kern._mask(ctx.mask)
         if kern.bb != "none" and m is None:
@@ -93,22 +98,26 @@ class _Rollout(torch.autograd.Function):
                                          _ptr(c(g_rho)), _ptr(c(g_u)), _ptr(c(g_p)), _ptr(gin),
                                          _ptr(fbar), _ptr(fbar_field), _ptr(work)),
                    "lbm_moments_adjoint")
-        # the recompute runs one step per launch on the reference state
-        stepper = copy.copy(kern)
-        stepper.scheme = "push"
+        # Recompute each segment with the forward's pull launches, keeping the
+        # post-collision states g_0 .. g_{n-2}; step k's starting state is the
+        # checkpoint for k = 0 and is gathered from g_{k-1} by the adjoint
+        # kernel otherwise.
         spare = torch.empty_like(fbar)
         nseg = len(ckpts)
         for s in reversed(range(nseg)):
             n = min(ctx.seg, ctx.nsteps - s * ctx.seg)
             states = [ckpts[s]]
-            for _ in range(n - 1):
-                states.append(stepper.step(states[-1], 1, mask=m, force=force,
-                                           tau_field=tau_field, moments=False)[0])
+            for k in range(n - 1):
+                g = torch.empty(kern.state_shape, dtype=kern.dtype, device=dev)
+                _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 0 if k == 0 else 1,
+                                            _ptr(states[-1]), _ptr(g), _ptr(m), _ptr(ff),
+                                            _ptr(tf), None, None, None), "lbm_pull_phase")
+                states.append(g)
             for i in reversed(range(n)):
-                _lib.check(L.lbm_adjoint_step(st, ctypes.byref(d), _ptr(states[i]), _ptr(fbar),
-                                              _ptr(spare), _ptr(m), _ptr(ff), _ptr(tf),
-                                              _ptr(tau_bar), _ptr(fbar_field), _ptr(work)),
-                           "lbm_adjoint_step")
+                _lib.check(L.lbm_adjoint_step(st, ctypes.byref(d), _ptr(states[i]), int(i > 0),
+                                              _ptr(fbar), _ptr(spare), _ptr(m), _ptr(ff),
+                                              _ptr(tf), _ptr(tau_bar), _ptr(fbar_field),
+                                              _ptr(work)), "lbm_adjoint_step")
                 fbar, spare = spare, fbar
             del states
         grad_force = None

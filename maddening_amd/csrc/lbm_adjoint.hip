@@ -15,6 +15,7 @@
 #include <string>
 
 #include "../../include/lbm_c_api.h"
+#include "lbm_cell.hpp"
 #include "lbm_common.hpp"
 #include "lbm_host.hpp"
 
@@ -39,6 +40,7 @@ struct AdjArgs {
   int bpr, nblocks, remap;
   C inv_tau;
   C Fa, Fb, Fc;
+  C lid_a, lid_b, lid_c;
 };
 
 // Sum three values over the workgroup and add them to partial[4*lb + k].
@@ -94,10 +96,12 @@ __device__ __forceinline__ void read_force(const AdjArgs<C>& a, int cell, C& Fa,
   }
 }
 
-template <class L, class T, int BB>
+// FPOST: `f` holds the post-collision state of the previous step and the
+// step's starting state f_n is gathered from it exactly as the forward pull
+// does; otherwise `f` holds f_n itself.
+template <class L, class T, int BB, bool FPOST>
 __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C> a) {
   using C = typename Storage<T>::C;
-  using St = Storage<T>;
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
   const int row = lb / a.bpr;
   const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
@@ -133,10 +137,9 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
         }
       }
       // forward quantities of this cell
-      const T* f = static_cast<const T*>(a.f);
       C d[L::Q];
-#pragma unroll
-      for (int q = 0; q < L::Q; ++q) d[q] = St::template load<L>(f + q * P + n.cell, q);
+      gather_state<L, T, BB, FPOST>(static_cast<const T*>(a.f), a.mask, P, n, self, a.lid_a,
+                                    a.lid_b, a.lid_c, d);
       C Fa, Fb, Fc;
       read_force<L, C>(a, n.cell, Fa, Fb, Fc);
       C drho, rho, ua, ub, uc;
@@ -307,14 +310,31 @@ AdjArgs<C> adj_args(const lbm_desc* d, const Geom& g) {
   a.inv_tau = C(1.0 / d->tau);
   if (d->q == 19) {
     a.Fa = C(d->force[0]); a.Fb = C(d->force[1]); a.Fc = C(d->force[2]);
+    a.lid_a = C(d->lid_velocity[0]); a.lid_b = C(d->lid_velocity[1]); a.lid_c = C(d->lid_velocity[2]);
   } else {
     a.Fa = C(0); a.Fb = C(d->force[0]); a.Fc = C(d->force[1]);
+    a.lid_a = C(0); a.lid_b = C(d->lid_velocity[0]); a.lid_c = C(d->lid_velocity[1]);
   }
   return a;
 }
 
+template <class L, class T, bool FPOST>
+int adjoint_launch(hipStream_t s, int bb, const Geom& g, const AdjArgs<typename Storage<T>::C>& a) {
+  switch (bb) {
+    case LBM_BB_NONE:
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_NONE, FPOST>), dim3(g.nblocks), dim3(256), 0, s, a);
+      break;
+    case LBM_BB_REFERENCE:
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_REF, FPOST>), dim3(g.nblocks), dim3(256), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF, FPOST>), dim3(g.nblocks), dim3(256), 0, s, a);
+  }
+  return check_launch("k_adjoint launch");
+}
+
 template <class L, class T>
-int adjoint_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f,
+int adjoint_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f, int f_post,
                   const void* gin, void* gout, const uint8_t* mask, const float* ff,
                   const float* tf, float* tau_bar, float* force_bar, void* work) {
   using C = typename Storage<T>::C;
@@ -328,17 +348,8 @@ int adjoint_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f
   a.tau_bar = tau_bar;
   a.force_bar = force_bar;
   a.partial = static_cast<double*>(work);
-  switch (d->bb_mode) {
-    case LBM_BB_NONE:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_NONE>), dim3(g.nblocks), dim3(256), 0, s, a);
-      break;
-    case LBM_BB_REFERENCE:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_REF>), dim3(g.nblocks), dim3(256), 0, s, a);
-      break;
-    default:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF>), dim3(g.nblocks), dim3(256), 0, s, a);
-  }
-  return check_launch("k_adjoint launch");
+  if (f_post) return adjoint_launch<L, T, true>(s, d->bb_mode, g, a);
+  return adjoint_launch<L, T, false>(s, d->bb_mode, g, a);
 }
 
 template <class L, class T>
@@ -388,10 +399,10 @@ int lbm_adjoint_begin(void* stream, const lbm_desc* d, void* work) {
   return check_launch("k_fill_zero launch");
 }
 
-int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, const void* fbar_next,
-                     void* fbar_out, const uint8_t* mask, const float* force_field,
-                     const float* tau_field, float* tau_bar, float* force_bar_field,
-                     void* work) {
+int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, int f_n_post,
+                     const void* fbar_next, void* fbar_out, const uint8_t* mask,
+                     const float* force_field, const float* tau_field, float* tau_bar,
+                     float* force_bar_field, void* work) {
   g_last_error.clear();
   (void)hipGetLastError();
   Geom g;
@@ -404,21 +415,21 @@ int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, const voi
   hipStream_t s = (hipStream_t)stream;
   if (d->q == 19) {
     if (d->storage == LBM_F32_DEV)
-      return adjoint_typed<Lattice<19>, f32dev_t>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+      return adjoint_typed<Lattice<19>, f32dev_t>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
                                                  tau_field, tau_bar, force_bar_field, work);
     if (d->storage == LBM_F32)
-      return adjoint_typed<Lattice<19>, float>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+      return adjoint_typed<Lattice<19>, float>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
                                               tau_field, tau_bar, force_bar_field, work);
-    return adjoint_typed<Lattice<19>, double>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+    return adjoint_typed<Lattice<19>, double>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
                                              tau_field, tau_bar, force_bar_field, work);
   }
   if (d->storage == LBM_F32_DEV)
-    return adjoint_typed<Lattice<9>, f32dev_t>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+    return adjoint_typed<Lattice<9>, f32dev_t>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
                                               tau_field, tau_bar, force_bar_field, work);
   if (d->storage == LBM_F32)
-    return adjoint_typed<Lattice<9>, float>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+    return adjoint_typed<Lattice<9>, float>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
                                            tau_field, tau_bar, force_bar_field, work);
-  return adjoint_typed<Lattice<9>, double>(s, d, g, f_n, fbar_next, fbar_out, mask, force_field,
+  return adjoint_typed<Lattice<9>, double>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
                                           tau_field, tau_bar, force_bar_field, work);
 }
 

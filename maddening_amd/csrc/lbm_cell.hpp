@@ -74,6 +74,53 @@ __device__ __forceinline__ C lid_term(const FwdArgs<C>& a, int q) {
          (C(L::ea[q]) * a.lid_a + C(L::eb[q]) * a.lid_b + C(L::ec[q]) * a.lid_c);
 }
 
+// The populations a cell holds after streaming and bounce-back, read from a
+// post-collision state `src` (PULL), or the populations stored at the cell
+// (!PULL). Returned as deviations d_q = f_q - w_q.
+template <class L, class T, int BB, bool PULL>
+__device__ __forceinline__ void gather_state(const T* src, const uint8_t* mask, long long P,
+                                             const Nbr& n, uint8_t self,
+                                             typename Storage<T>::C lid_a,
+                                             typename Storage<T>::C lid_b,
+                                             typename Storage<T>::C lid_c,
+                                             typename Storage<T>::C (&d)[L::Q]) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  if (BB == BB_HALF && PULL && self) {
+    // solid cell under halfway bounce-back: no fluid reads it; park it at rest
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) d[q] = C(0);
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    if (!PULL) {
+      d[q] = St::template load<L>(src + q * P + n.cell, q);
+    } else if (BB == BB_REF) {
+      // reference bounce-back (lbm.py:944-948): at a wall cell slot q takes
+      // the value streamed into slot opp(q), which came from x + e_q.
+      const T* ptr = self ? src + L::opp[q] * P + n.template at<L>(q, +1)
+                          : src + q * P + n.template at<L>(q, -1);
+      d[q] = St::template load<L>(ptr, q);
+    } else if (BB == BB_HALF) {
+      const int s = n.template at<L>(q, -1);
+      const uint8_t m = (q == 0) ? uint8_t(0) : mask[s];
+      if (m) {
+        // 2 w_q (e_q . u_wall) / cs2: momentum a moving wall (mask 2) adds to
+        // the population it reflects into direction q (wall density 1)
+        const C lid = m == 2 ? C(6) * weight<L, C>(q) *
+                                   (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c)
+                             : C(0);
+        d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) + lid;
+      } else {
+        d[q] = St::template load<L>(src + q * P + s, q);
+      }
+    } else {
+      d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
+    }
+  }
+}
+
 // PULL: gather (stream + bounce-back) instead of an aligned read.
 // COLLIDE: apply the collision after the read.
 // STORE: write the state back (false for the moments-only pass).
@@ -91,35 +138,7 @@ __device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C
   const uint8_t self = a.mask ? a.mask[n.cell] : uint8_t(0);
 
   C d[L::Q];
-  if (BB == BB_HALF && PULL && self) {
-    // solid cell under halfway bounce-back: no fluid reads it; park it at rest
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) d[q] = C(0);
-  } else {
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) {
-      if (!PULL) {
-        d[q] = St::template load<L>(src + q * P + n.cell, q);
-      } else if (BB == BB_REF) {
-        // reference bounce-back (lbm.py:944-948): at a wall cell slot q takes
-        // the value streamed into slot opp(q), which came from x + e_q.
-        const T* ptr = self ? src + L::opp[q] * P + n.template at<L>(q, +1)
-                            : src + q * P + n.template at<L>(q, -1);
-        d[q] = St::template load<L>(ptr, q);
-      } else if (BB == BB_HALF) {
-        const int s = n.template at<L>(q, -1);
-        const uint8_t m = (q == 0) ? uint8_t(0) : a.mask[s];
-        if (m) {
-          d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) +
-                 (m == 2 ? lid_term<L, C>(a, q) : C(0));
-        } else {
-          d[q] = St::template load<L>(src + q * P + s, q);
-        }
-      } else {
-        d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
-      }
-    }
-  }
+  gather_state<L, T, BB, PULL>(src, a.mask, P, n, self, a.lid_a, a.lid_b, a.lid_c, d);
 
   C Fa, Fb, Fc;
   load_force<L, C>(a, n.cell, Fa, Fb, Fc);

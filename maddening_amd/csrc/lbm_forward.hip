@@ -517,6 +517,39 @@ struct StepOp {
   }
 };
 
+template <class L, class T, int BB>
+int pull_phase_bb(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>::C>& a,
+                  int phase) {
+  if (phase == 0) return launch_cell<L, T, BB, false, true, true>(s, g, a);
+  if (phase == 1) return launch_cell<L, T, BB, true, true, true>(s, g, a);
+  return launch_cell<L, T, BB, true, false, true>(s, g, a);
+}
+
+template <class L, class T>
+struct PullPhaseOp {
+  static int run(hipStream_t s, const lbm_desc* d, const Geom& g, int phase, const void* src,
+                 void* dst, const uint8_t* mask, const float* ff, const float* tf, float* rho,
+                 float* u, float* p) {
+    using C = typename Storage<T>::C;
+    FwdArgs<C> a = make_args<C>(d, g);
+    a.src = src;
+    a.dst = dst;
+    a.mask = d->bb_mode == LBM_BB_NONE ? nullptr : mask;
+    a.force_field = ff;
+    a.tau_field = tf;
+    if (phase == 2) {
+      a.rho = rho; a.u = u; a.p = p;
+    }
+    Geom gr = g;
+    gr.rowv = row_width(d, g);
+    switch (d->bb_mode) {
+      case LBM_BB_NONE: return pull_phase_bb<L, T, BB_NONE>(s, gr, a, phase);
+      case LBM_BB_REFERENCE: return pull_phase_bb<L, T, BB_REF>(s, gr, a, phase);
+      default: return pull_phase_bb<L, T, BB_HALF>(s, gr, a, phase);
+    }
+  }
+};
+
 template <class L, class T>
 struct MomentsOp {
   static int run(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f,
@@ -631,6 +664,23 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out, voi
     return fail(LBM_ERR_INVALID, "work must not alias f_in or f_out");
   return dispatch<StepOp>(d, (hipStream_t)stream, d, g, f_in, f_out, work, mask, force_field,
                           tau_field, rho, u, p, nsteps, scheme);
+}
+
+int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, void* dst,
+                   const uint8_t* mask, const float* force_field, const float* tau_field,
+                   float* rho, float* u, float* p) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (phase < 0 || phase > 2) return fail(LBM_ERR_INVALID, "phase must be 0, 1 or 2");
+  if (!src || !dst) return fail(LBM_ERR_INVALID, "src and dst must be non-NULL");
+  if (src == dst) return fail(LBM_ERR_INVALID, "src and dst must be distinct buffers");
+  if (d->bb_mode != LBM_BB_NONE && !mask)
+    return fail(LBM_ERR_INVALID, "bb_mode needs a wall mask");
+  return dispatch<PullPhaseOp>(d, (hipStream_t)stream, d, g, phase, src, dst, mask, force_field,
+                               tau_field, rho, u, p);
 }
 
 int lbm_moments(void* stream, const lbm_desc* d, const void* f, const uint8_t* mask,
