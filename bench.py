@@ -269,10 +269,10 @@ def batch_leg(dev, nsteps=200, seed=0):
             "tuned": {"cells_per_thread": tune["cells_per_thread"], "grid": tune["grid"]}}
 
 
-def read_traffic():
+def read_traffic(storage):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", "pmc_d3q19_256_f32.json")
+    path = os.path.join(ROOT, "profiles", f"pmc_d3q19_256_{storage}.json")
     try:
         with open(path) as fh:
             return json.load(fh).get("traffic_bytes_per_launch")
@@ -358,8 +358,8 @@ def main():
                    "cells": cells, "scheme": kern.scheme, "wall_s_timed_call": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(),
-                     "kernel": "k_cell<D3Q19,f32,pull+collide> (152 B/cell algorithmic)",
+                     "traffic": read_traffic(args.storage),
+                     "kernel": f"k_cell<D3Q19,{args.storage},pull+collide> (152 B/cell algorithmic)",
                      "avg_launch_us": round(per_launch_s * 1e6, 2)},
     }
     extra = {}

@@ -172,7 +172,8 @@ int lbm_conservation(void* stream, const lbm_desc* d, const void* f,
  * No counterpart in the reference: its only batching is jax.vmap over
  * equal-shaped states in GraphManager.run_sweep (graph_manager.py:2807-2877).
  * Here lattices of any size, lattice type, tau, force and wall rule advance
- * together: one launch per step covers all of them (float32 storage).
+ * together (float32 storage): each step is one launch per lattice class, and
+ * a whole run can be replayed as one HIP graph.
  */
 typedef struct lbm_batch_item {
   int32_t q;              /* 9 or 19 */
@@ -188,27 +189,36 @@ typedef struct lbm_batch_item {
   float* p;
 } lbm_batch_item;
 
-/* Bytes of device memory for the plan of `nitems` lattices. */
+/* Opaque host-side plan: the class ranges the launches need. */
+typedef struct lbm_batch_plan_t lbm_batch_plan_t;
+
+/* Bytes of device memory for the plan table of `nitems` lattices. */
 size_t lbm_batch_plan_size(int nitems);
 
 /*
- * Build the plan: validates every item, cuts each lattice into tiles of
- * 256 * cells_per_thread consecutive cells (1..16; 0 = 1) and uploads the
- * table to `plan`. A set-up call: it waits for the upload on `stream`.
- * *total_tiles (nullable) receives the tile count.
+ * Build a plan: validates every item, orders the items into classes of equal
+ * (q, bb_mode), cuts each lattice into tiles of 256 * cells_per_thread
+ * consecutive cells (1..16; 0 = 1) and uploads the table to `device_plan`
+ * (lbm_batch_plan_size bytes, owned by the caller, alive as long as the
+ * plan). A set-up call: it waits for the upload on `stream`. *total_tiles
+ * (nullable) receives the tile count. Free the handle with
+ * lbm_batch_plan_destroy.
  */
-int lbm_batch_plan(void* stream, const lbm_batch_item* items, int nitems,
-                   int cells_per_thread, void* plan, long long* total_tiles);
+int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
+                          int cells_per_thread, void* device_plan, lbm_batch_plan_t** plan,
+                          long long* total_tiles);
+int lbm_batch_plan_destroy(lbm_batch_plan_t* plan);
 
 /*
- * nsteps LBMNode.update steps of every lattice in the plan, one launch of
- * `grid` persistent workgroups per step (nsteps + 1 launches: the pull
- * sequence). The final state of each item is in f[(nsteps + 1) % 2].
+ * nsteps LBMNode.update steps of every lattice in the plan: per step one
+ * launch per lattice class present (at most 6), each striding `grid`
+ * workgroups over that class's tiles (nsteps + 1 rounds: the pull sequence).
+ * The final state of each item is in f[(nsteps + 1) % 2].
  */
-int lbm_batch_run(void* stream, const void* plan, int nsteps, int grid);
+int lbm_batch_run(void* stream, const lbm_batch_plan_t* plan, int nsteps, int grid);
 
 /* The same run captured once into a HIP graph; launch it as often as needed. */
-int lbm_batch_graph_create(const void* plan, int nsteps, int grid, void** graph);
+int lbm_batch_graph_create(const lbm_batch_plan_t* plan, int nsteps, int grid, void** graph);
 int lbm_batch_graph_launch(void* stream, void* graph);
 int lbm_batch_graph_destroy(void* graph);
 

@@ -75,8 +75,10 @@ _PROTOS = {
                                    _vp, _vp]),
     "lbm_reduce_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(LbmDesc)]),
     "lbm_batch_plan_size": (ctypes.c_size_t, [ctypes.c_int]),
-    "lbm_batch_plan": (ctypes.c_int, [_vp, ctypes.POINTER(LbmBatchItem), ctypes.c_int, ctypes.c_int,
-                                      _vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "lbm_batch_plan_create": (ctypes.c_int, [_vp, ctypes.POINTER(LbmBatchItem), ctypes.c_int,
+                                             ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_void_p),
+                                             ctypes.POINTER(ctypes.c_longlong)]),
+    "lbm_batch_plan_destroy": (ctypes.c_int, [_vp]),
     "lbm_batch_run": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
     "lbm_batch_graph_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_void_p)]),

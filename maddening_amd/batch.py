@@ -45,6 +45,7 @@ class LatticeBatch:
                               torch.as_tensor(np.asarray(m, np.uint8), device=self.device))
         self.cells = sum(k.ncells for k in self.kernels)
         self._plan = None
+        self._handle = None
         self._graphs = []
         self.cells_per_thread = 0
         self.total_tiles = 0
@@ -77,10 +78,13 @@ class LatticeBatch:
         n = len(self.kernels)
         plan = torch.empty(L.lbm_batch_plan_size(n), dtype=torch.uint8, device=self.device)
         tiles = ctypes.c_longlong(0)
+        handle = ctypes.c_void_p()
         items = self._items()
-        _lib.check(L.lbm_batch_plan(_stream(self.device), items, n, int(cells_per_thread),
-                                    _ptr(plan), ctypes.byref(tiles)), "lbm_batch_plan")
+        _lib.check(L.lbm_batch_plan_create(_stream(self.device), items, n, int(cells_per_thread),
+                                           _ptr(plan), ctypes.byref(handle), ctypes.byref(tiles)),
+                   "lbm_batch_plan_create")
         self._plan = plan
+        self._handle = handle
         self.cells_per_thread = int(cells_per_thread)
         self.total_tiles = int(tiles.value)
 
@@ -101,14 +105,14 @@ class LatticeBatch:
     # -- execution -------------------------------------------------------------
     def run(self, nsteps: int, grid: Optional[int] = None) -> None:
         g = grid or self.default_grid()
-        _lib.check(_lib.lib().lbm_batch_run(_stream(self.device), _ptr(self._plan), int(nsteps),
+        _lib.check(_lib.lib().lbm_batch_run(_stream(self.device), self._handle, int(nsteps),
                                             int(g)), "lbm_batch_run")
 
     def graph(self, nsteps: int, grid: Optional[int] = None) -> ctypes.c_void_p:
         """Capture a run of ``nsteps`` into a HIP graph (freed with the batch)."""
         g = grid or self.default_grid()
         h = ctypes.c_void_p()
-        _lib.check(_lib.lib().lbm_batch_graph_create(_ptr(self._plan), int(nsteps), int(g),
+        _lib.check(_lib.lib().lbm_batch_graph_create(self._handle, int(nsteps), int(g),
                                                      ctypes.byref(h)), "lbm_batch_graph_create")
         self._graphs.append(h)
         return h
@@ -128,9 +132,13 @@ class LatticeBatch:
         return outs
 
     def close(self) -> None:
+        """Release captured graphs and the plan handle."""
         for h in self._graphs:
             _lib.lib().lbm_batch_graph_destroy(h)
         self._graphs = []
+        if self._handle is not None:
+            _lib.lib().lbm_batch_plan_destroy(self._handle)
+            self._handle = None
 
     def __del__(self):
         try:

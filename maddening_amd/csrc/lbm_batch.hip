@@ -14,6 +14,7 @@
 // A whole n-step run can be captured once into a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -51,23 +52,19 @@ __host__ __device__ inline const BatchEntry* entries(const void* plan) {
   return reinterpret_cast<const BatchEntry*>(static_cast<const char*>(plan) + kHeaderBytes);
 }
 
-template <class L, bool PULL, bool COLLIDE>
-__device__ __forceinline__ void batch_cell(const FwdArgs<float>& a, int bb, int row, int ic) {
-  if (bb == BB_NONE) cell_update<L, float, BB_NONE, PULL, COLLIDE, true>(a, row, ic);
-  else if (bb == BB_REF) cell_update<L, float, BB_REF, PULL, COLLIDE, true>(a, row, ic);
-  else cell_update<L, float, BB_HALF, PULL, COLLIDE, true>(a, row, ic);
-}
-
-// PHASE 0: collide only; 1: stream + collide; 2: stream (+ moments)
-template <int PHASE>
-__global__ void __launch_bounds__(256) k_batch(const void* plan, int parity) {
+// PHASE 0: collide only; 1: stream + collide; 2: stream (+ moments).
+// One launch covers the tiles [t0, t1) of the entries [e0, e1), which all
+// share the lattice L and wall rule BB.
+template <class L, int BB, int PHASE>
+__global__ void __launch_bounds__(256)
+    k_batch(const void* plan, int parity, long long t0, long long t1, int e0, int e1) {
   const BatchHeader* h = static_cast<const BatchHeader*>(plan);
   const BatchEntry* items = entries(plan);
-  const long long total = h->total_tiles;
-  const int n = h->nitems;
   const int cpt = h->cells_per_thread;
-  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
-    int lo = 0, hi = n - 1;
+  constexpr bool PULL = PHASE != 0;
+  constexpr bool COLLIDE = PHASE != 2;
+  for (long long t = t0 + blockIdx.x; t < t1; t += gridDim.x) {
+    int lo = e0, hi = e1 - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (items[mid].tile_begin <= t) lo = mid;
@@ -87,32 +84,77 @@ __global__ void __launch_bounds__(256) k_batch(const void* plan, int parity) {
     if (PHASE == 2) {
       a.rho = e.rho; a.u = e.u; a.p = e.p;
     }
-    constexpr bool PULL = PHASE != 0;
-    constexpr bool COLLIDE = PHASE != 2;
     for (int k = 0; k < cpt; ++k) {
       const long long cell = base + 256LL * k;
       if (cell >= e.plane) break;
       const int row = (int)(cell / e.nc);
       const int ic = (int)(cell - (long long)row * e.nc);
-      if (e.q == 19) batch_cell<Lattice<19>, PULL, COLLIDE>(a, e.bb, row, ic);
-      else batch_cell<Lattice<9>, PULL, COLLIDE>(a, e.bb, row, ic);
+      cell_update<L, float, BB, PULL, COLLIDE, true>(a, row, ic);
     }
   }
 }
 
-inline int batch_launch(hipStream_t s, const void* plan, int phase, int parity, int grid) {
+// A lattice class: entries [e0, e1) with tiles [t0, t1), same (q, wall rule).
+struct BatchGroup {
+  int q, bb;
+  int e0, e1;
+  long long t0, t1;
+};
+
+template <class L, int BB>
+int group_launch(hipStream_t s, const void* dplan, const BatchGroup& gr, int phase, int parity,
+                 int grid) {
+  const long long tiles = gr.t1 - gr.t0;
+  const unsigned g = (unsigned)(grid < tiles ? grid : tiles);
   switch (phase) {
-    case 0: hipLaunchKernelGGL(k_batch<0>, dim3(grid), dim3(256), 0, s, plan, parity); break;
-    case 1: hipLaunchKernelGGL(k_batch<1>, dim3(grid), dim3(256), 0, s, plan, parity); break;
-    default: hipLaunchKernelGGL(k_batch<2>, dim3(grid), dim3(256), 0, s, plan, parity); break;
+    case 0:
+      hipLaunchKernelGGL((k_batch<L, BB, 0>), dim3(g), dim3(256), 0, s, dplan, parity, gr.t0,
+                         gr.t1, gr.e0, gr.e1);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_batch<L, BB, 1>), dim3(g), dim3(256), 0, s, dplan, parity, gr.t0,
+                         gr.t1, gr.e0, gr.e1);
+      break;
+    default:
+      hipLaunchKernelGGL((k_batch<L, BB, 2>), dim3(g), dim3(256), 0, s, dplan, parity, gr.t0,
+                         gr.t1, gr.e0, gr.e1);
   }
   return check_launch("k_batch launch");
 }
 
-inline int batch_run(hipStream_t s, const void* plan, int nsteps, int grid) {
-  int rc = batch_launch(s, plan, 0, 0, grid);
-  for (int k = 1; k < nsteps && rc == LBM_OK; ++k) rc = batch_launch(s, plan, 1, k & 1, grid);
-  if (rc == LBM_OK) rc = batch_launch(s, plan, 2, nsteps & 1, grid);
+}  // namespace mlbm
+
+// Host handle of a plan: the device table plus the class ranges the
+// launches need (host memory; the device table is the caller's buffer).
+struct lbm_batch_plan_t {
+  const void* dplan;
+  std::vector<mlbm::BatchGroup> groups;
+};
+
+namespace mlbm {
+
+inline int batch_launch(hipStream_t s, const lbm_batch_plan_t* p, int phase, int parity,
+                        int grid) {
+  for (const BatchGroup& gr : p->groups) {
+    int rc;
+    if (gr.q == 19) {
+      if (gr.bb == BB_NONE) rc = group_launch<Lattice<19>, BB_NONE>(s, p->dplan, gr, phase, parity, grid);
+      else if (gr.bb == BB_REF) rc = group_launch<Lattice<19>, BB_REF>(s, p->dplan, gr, phase, parity, grid);
+      else rc = group_launch<Lattice<19>, BB_HALF>(s, p->dplan, gr, phase, parity, grid);
+    } else {
+      if (gr.bb == BB_NONE) rc = group_launch<Lattice<9>, BB_NONE>(s, p->dplan, gr, phase, parity, grid);
+      else if (gr.bb == BB_REF) rc = group_launch<Lattice<9>, BB_REF>(s, p->dplan, gr, phase, parity, grid);
+      else rc = group_launch<Lattice<9>, BB_HALF>(s, p->dplan, gr, phase, parity, grid);
+    }
+    if (rc) return rc;
+  }
+  return LBM_OK;
+}
+
+inline int batch_run(hipStream_t s, const lbm_batch_plan_t* p, int nsteps, int grid) {
+  int rc = batch_launch(s, p, 0, 0, grid);
+  for (int k = 1; k < nsteps && rc == LBM_OK; ++k) rc = batch_launch(s, p, 1, k & 1, grid);
+  if (rc == LBM_OK) rc = batch_launch(s, p, 2, nsteps & 1, grid);
   return rc;
 }
 
@@ -131,17 +173,20 @@ size_t lbm_batch_plan_size(int nitems) {
   return kHeaderBytes + sizeof(BatchEntry) * (size_t)nitems;
 }
 
-int lbm_batch_plan(void* stream, const lbm_batch_item* items, int nitems,
-                   int cells_per_thread, void* plan, long long* total_tiles) {
+int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
+                          int cells_per_thread, void* device_plan, lbm_batch_plan_t** plan,
+                          long long* total_tiles) {
   g_last_error.clear();
   (void)hipGetLastError();
-  if (!items || nitems < 1 || !plan) return fail(LBM_ERR_INVALID, "items, nitems >= 1 and plan needed");
+  if (!items || nitems < 1 || !device_plan || !plan)
+    return fail(LBM_ERR_INVALID, "items, nitems >= 1, device_plan and plan needed");
+  *plan = nullptr;
   const int cpt = cells_per_thread == 0 ? 1 : cells_per_thread;
   if (cpt < 1 || cpt > 16) return fail(LBM_ERR_INVALID, "cells_per_thread must be 1..16");
   const long long tile_cells = 256LL * cpt;
-  std::vector<char> host(lbm_batch_plan_size(nitems), 0);
-  BatchEntry* out = reinterpret_cast<BatchEntry*>(host.data() + kHeaderBytes);
-  long long tiles = 0;
+  // validate, then order the entries by (q, wall rule) so each class is a
+  // contiguous run of entries and tiles
+  std::vector<Geom> geo(nitems);
   for (int i = 0; i < nitems; ++i) {
     const lbm_batch_item& it = items[i];
     lbm_desc d = {};
@@ -150,14 +195,29 @@ int lbm_batch_plan(void* stream, const lbm_batch_item* items, int nitems,
     d.bb_mode = it.bb_mode;
     d.tau = it.tau;
     for (int k = 0; k < 3; ++k) d.dims[k] = it.dims[k];
-    Geom g;
-    int rc = check_desc(&d, &g);
+    int rc = check_desc(&d, &geo[i]);
     if (rc) return fail(rc, "batch item " + std::to_string(i) + ": " + g_last_error);
     if (!it.f[0] || !it.f[1] || it.f[0] == it.f[1])
       return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": needs two distinct state buffers");
     if (it.bb_mode != LBM_BB_NONE && !it.mask)
       return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": bb_mode needs a mask");
-    BatchEntry& e = out[i];
+  }
+  std::vector<int> order(nitems);
+  for (int i = 0; i < nitems; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const int kx = items[x].q * 4 + items[x].bb_mode, ky = items[y].q * 4 + items[y].bb_mode;
+    return kx < ky;
+  });
+  std::vector<char> host(lbm_batch_plan_size(nitems), 0);
+  BatchEntry* out = reinterpret_cast<BatchEntry*>(host.data() + kHeaderBytes);
+  lbm_batch_plan_t* hp = new lbm_batch_plan_t;
+  hp->dplan = device_plan;
+  long long tiles = 0;
+  for (int j = 0; j < nitems; ++j) {
+    const int i = order[j];
+    const lbm_batch_item& it = items[i];
+    const Geom& g = geo[i];
+    BatchEntry& e = out[j];
     e.q = it.q;
     e.bb = it.bb_mode;
     e.na = g.na; e.nb = g.nb; e.nc = g.nc;
@@ -179,7 +239,12 @@ int lbm_batch_plan(void* stream, const lbm_batch_item* items, int nitems,
     e.rho = it.rho;
     e.u = it.u;
     e.p = it.p;
-    tiles += (g.plane + tile_cells - 1) / tile_cells;
+    const long long nt = (g.plane + tile_cells - 1) / tile_cells;
+    if (hp->groups.empty() || hp->groups.back().q != it.q || hp->groups.back().bb != it.bb_mode)
+      hp->groups.push_back(BatchGroup{it.q, it.bb_mode, j, j, tiles, tiles});
+    hp->groups.back().e1 = j + 1;
+    hp->groups.back().t1 = tiles + nt;
+    tiles += nt;
   }
   BatchHeader* hdr = reinterpret_cast<BatchHeader*>(host.data());
   hdr->total_tiles = tiles;
@@ -187,20 +252,29 @@ int lbm_batch_plan(void* stream, const lbm_batch_item* items, int nitems,
   hdr->cells_per_thread = cpt;
   if (total_tiles) *total_tiles = tiles;
   hipStream_t s = (hipStream_t)stream;
-  hipError_t err = hipMemcpyAsync(plan, host.data(), host.size(), hipMemcpyHostToDevice, s);
+  hipError_t err = hipMemcpyAsync(device_plan, host.data(), host.size(), hipMemcpyHostToDevice, s);
   if (err == hipSuccess) err = hipStreamSynchronize(s);  // the host table dies with this call
-  if (err != hipSuccess) return fail(LBM_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(err));
+  if (err != hipSuccess) {
+    delete hp;
+    return fail(LBM_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(err));
+  }
+  *plan = hp;
   return LBM_OK;
 }
 
-int lbm_batch_run(void* stream, const void* plan, int nsteps, int grid) {
+int lbm_batch_plan_destroy(lbm_batch_plan_t* plan) {
+  delete plan;
+  return LBM_OK;
+}
+
+int lbm_batch_run(void* stream, const lbm_batch_plan_t* plan, int nsteps, int grid) {
   g_last_error.clear();
   (void)hipGetLastError();
   if (!plan || nsteps < 1 || grid < 1) return fail(LBM_ERR_INVALID, "plan, nsteps >= 1 and grid >= 1 needed");
   return batch_run((hipStream_t)stream, plan, nsteps, grid);
 }
 
-int lbm_batch_graph_create(const void* plan, int nsteps, int grid, void** graph) {
+int lbm_batch_graph_create(const lbm_batch_plan_t* plan, int nsteps, int grid, void** graph) {
   g_last_error.clear();
   (void)hipGetLastError();
   if (!plan || nsteps < 1 || grid < 1 || !graph)
