@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LBM_API_VERSION 1
+#define LBM_API_VERSION 2
 
 typedef enum lbm_status {
   LBM_OK = 0,
@@ -81,6 +81,13 @@ typedef struct lbm_desc {
                            0 = auto, -1 = never */
   int32_t flags;        /* LBM_FLAG_* */
   int32_t reserved;     /* must be 0 */
+  /* Zou-He pressure faces (lbm.py:425-524, 950-968), applied after streaming,
+   * inlet first: 0 = none, 1..6 = x_min, x_max, y_min, y_max, z_min, z_max
+   * in the reference's axis order (_FACE_MAP, lbm.py:517-524). */
+  int32_t inlet_face;
+  int32_t outlet_face;
+  double inlet_rho;     /* prescribed density = inlet_pressure / cs2 */
+  double outlet_rho;
 } lbm_desc;
 
 /* flags */

@@ -41,6 +41,10 @@ class LbmDesc(ctypes.Structure):
         ("row_vec", ctypes.c_int32),
         ("flags", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("inlet_face", ctypes.c_int32),
+        ("outlet_face", ctypes.c_int32),
+        ("inlet_rho", ctypes.c_double),
+        ("outlet_rho", ctypes.c_double),
     ]
 
 

@@ -288,6 +288,8 @@ inline int adj_geom(const lbm_desc* d, Geom* g) {
   if (rc) return rc;
   if (d->storage != LBM_F32 && d->storage != LBM_F64 && d->storage != LBM_F32_DEV)
     return fail(LBM_ERR_UNSUPPORTED, "the adjoint needs f32, f32dev or f64 storage");
+  if (d->inlet_face > 0 || d->outlet_face > 0)
+    return fail(LBM_ERR_UNSUPPORTED, "the adjoint does not cover Zou-He pressure faces");
   g->bx = 256;
   g->bpr = (g->nc + 255) / 256;
   g->nblocks = g->na * g->nb * g->bpr;

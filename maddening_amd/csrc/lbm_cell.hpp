@@ -27,7 +27,52 @@ struct FwdArgs {
   C inv_tau;
   C Fa, Fb, Fc;
   C lid_a, lid_b, lid_c;
+  // Zou-He pressure faces applied after streaming, in order (inlet, outlet)
+  int zh_n;
+  int zh_axis[2];  // internal axis 0 (a), 1 (b), 2 (c)
+  int zh_side[2];  // 0: first layer, 1: last layer
+  C zh_rho[2];     // prescribed density P / cs2
 };
+
+// Zou-He pressure face at one fluid cell (lbm.py:425-510), on deviations.
+// Directions pointing out of the domain through the face are unknown; the
+// normal velocity follows the reference's form
+//   u_n = +-(1 - (sum known + sum tangential) / rho_p)
+// and each unknown population is rebuilt by non-equilibrium bounce-back,
+// f_q = f_opp(q) + feq_q - feq_opp(q) = f_opp(q) + 6 w_q rho_p e_qn u_n.
+// Only unknown slots are written, and they read known slots: in place is safe.
+template <class L, class C>
+__device__ __forceinline__ void zou_he(C (&d)[L::Q], int axis, int side, C rho_p) {
+  C S = 0;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int e = axis == 0 ? L::ea[q] : (axis == 1 ? L::eb[q] : L::ec[q]);
+    if (side == 0 ? e <= 0 : e >= 0) S += d[q] + weight<L, C>(q);
+  }
+  const C r = rho_p > C(1e-10) ? rho_p : C(1e-10);
+  const C un = side == 0 ? C(1) - S / r : C(-1) + S / r;
+  C nd[L::Q];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int e = axis == 0 ? L::ea[q] : (axis == 1 ? L::eb[q] : L::ec[q]);
+    const bool unknown = side == 0 ? e > 0 : e < 0;
+    nd[q] = unknown ? d[L::opp[q]] + C(6) * weight<L, C>(q) * rho_p * C(e) * un : d[q];
+  }
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) d[q] = nd[q];
+}
+
+// Apply the configured faces to a cell at (ia, ib, ic) that is not a wall.
+template <class L, class C>
+__device__ __forceinline__ void apply_faces(const FwdArgs<C>& a, int ia, int ib, int ic,
+                                            C (&d)[L::Q]) {
+  for (int i = 0; i < a.zh_n; ++i) {
+    const int ax = a.zh_axis[i];
+    const int coord = ax == 0 ? ia : (ax == 1 ? ib : ic);
+    const int n = ax == 0 ? a.na : (ax == 1 ? a.nb : a.nc);
+    if (coord == (a.zh_side[i] == 0 ? 0 : n - 1)) zou_he<L, C>(d, ax, a.zh_side[i], a.zh_rho[i]);
+  }
+}
 
 template <class L, class C>
 __device__ __forceinline__ void load_force(const FwdArgs<C>& a, int cell, C& Fa,
@@ -139,6 +184,7 @@ __device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C
 
   C d[L::Q];
   gather_state<L, T, BB, PULL>(src, a.mask, P, n, self, a.lid_a, a.lid_b, a.lid_c, d);
+  if (PULL && a.zh_n && !self) apply_faces<L, C>(a, ia, ib, ic, d);
 
   C Fa, Fb, Fc;
   load_force<L, C>(a, n.cell, Fa, Fb, Fc);

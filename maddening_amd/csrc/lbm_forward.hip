@@ -226,6 +226,37 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
   for (int q = 0; q < L::Q; ++q) vstore_nt<T, V>(dst + q * P + cell0, out[q]);
 }
 
+// Zou-He face `which` applied in place to a post-stream state (the push
+// path, where the face cells' populations only exist after the scatter).
+// One thread per cell of the face.
+template <class L, class T>
+__global__ void __launch_bounds__(256) k_face(FwdArgs<typename Storage<T>::C> a, int which) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  const int ax = a.zh_axis[which];
+  const int side = a.zh_side[which];
+  const int n1 = ax == 0 ? a.nb : a.na;
+  const int n2 = ax == 2 ? a.nb : a.nc;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n1 * n2) return;
+  const int u = (int)(t / n2);
+  const int v = (int)(t - (long long)u * n2);
+  const int nax = ax == 0 ? a.na : (ax == 1 ? a.nb : a.nc);
+  const int coord = side == 0 ? 0 : nax - 1;
+  const int ia = ax == 0 ? coord : u;
+  const int ib = ax == 0 ? u : (ax == 1 ? coord : v);
+  const int ic = ax == 2 ? coord : v;
+  const long long cell = ((long long)ia * a.nb + ib) * a.nc + ic;
+  if (a.mask && a.mask[cell]) return;
+  T* f = static_cast<T*>(a.dst);
+  C d[L::Q];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) d[q] = St::template load<L>(f + q * a.plane + cell, q);
+  zou_he<L, C>(d, ax, side, a.zh_rho[which]);
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) St::template store<L>(f + q * a.plane + cell, q, d[q]);
+}
+
 // f = feq(rho, u) (lbm.py:180-206) written as deviations.
 template <class L, class T>
 __global__ void __launch_bounds__(256)
@@ -369,6 +400,17 @@ FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
     a.Fa = C(0); a.Fb = C(d->force[0]); a.Fc = C(d->force[1]);
     a.lid_a = C(0); a.lid_b = C(d->lid_velocity[0]); a.lid_c = C(d->lid_velocity[1]);
   }
+  const int faces[2] = {d->inlet_face, d->outlet_face};
+  const double rhos[2] = {d->inlet_rho, d->outlet_rho};
+  a.zh_n = 0;
+  for (int i = 0; i < 2; ++i) {
+    if (faces[i] <= 0) continue;
+    const int ref_axis = (faces[i] - 1) / 2;  // 1..6: x_min, x_max, y_min, y_max, z_min, z_max
+    a.zh_axis[a.zh_n] = ref_axis + (d->q == 19 ? 0 : 1);
+    a.zh_side[a.zh_n] = (faces[i] - 1) % 2;
+    a.zh_rho[a.zh_n] = C(rhos[i]);
+    ++a.zh_n;
+  }
   return a;
 }
 
@@ -380,6 +422,7 @@ FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
 // 5.4 vs 4.0 TB/s f16), so auto (row_vec 0) keeps it off.
 inline int row_width(const lbm_desc* d, const Geom& g) {
   if (d->bb_mode != LBM_BB_NONE || d->row_vec <= 0) return 0;
+  if (d->inlet_face > 0 || d->outlet_face > 0) return 0;  // faces live in the per-cell kernels
   return g.nc % (64 * d->row_vec) == 0 ? d->row_vec : 0;
 }
 
@@ -451,6 +494,13 @@ int step_push(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
     k.src = cur;
     k.dst = buf[(nsteps - 1 - i) & 1];
     if ((rc = launch_push<L, T, BB>(s, g, k))) return rc;
+    for (int face = 0; face < k.zh_n; ++face) {
+      const int ax = k.zh_axis[face];
+      const long long cells = (long long)(ax == 0 ? g.nb : g.na) * (ax == 2 ? g.nb : g.nc);
+      hipLaunchKernelGGL((k_face<L, T>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, k,
+                         face);
+      if ((rc = check_launch("k_face launch"))) return rc;
+    }
     cur = k.dst;
   }
   if (rho || u || p) {

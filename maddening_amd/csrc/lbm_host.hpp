@@ -73,6 +73,13 @@ inline int check_desc(const lbm_desc* d, Geom* g) {
     return fail(LBM_ERR_UNSUPPORTED, "row_vec 4 is not built for f64 storage (use 1 or 2)");
   if (d->row_vec > 0 && d->storage == LBM_F32_DEV)
     return fail(LBM_ERR_UNSUPPORTED, "the row-per-wave kernels are not built for f32dev storage");
+  const int D = d->q == 19 ? 3 : 2;
+  for (int face : {d->inlet_face, d->outlet_face}) {
+    if (face < 0 || face > 6)
+      return fail(LBM_ERR_INVALID, "inlet_face/outlet_face must be 0 (none) or 1..6");
+    if (face > 0 && (face - 1) / 2 >= D)
+      return fail(LBM_ERR_INVALID, "a pressure face uses an axis the lattice does not have");
+  }
   if (d->row_vec > 0 && g->nc % (64 * d->row_vec) != 0)
     return fail(LBM_ERR_INVALID, "row_vec needs the contiguous axis to be a multiple of 64*row_vec");
   g->bx = bx;

@@ -102,7 +102,11 @@ class LBMKernel:
         _lib.lib()  # fail loudly now if the library is missing
 
     # -- descriptors -----------------------------------------------------
-    def desc(self, force=None, tau: Optional[float] = None) -> _lib.LbmDesc:
+    FACES = ("x_min", "x_max", "y_min", "y_max", "z_min", "z_max")
+
+    def desc(self, force=None, tau: Optional[float] = None, faces=None) -> _lib.LbmDesc:
+        """Descriptor for a call. ``faces``: optional ((inlet_face, inlet_rho),
+        (outlet_face, outlet_rho)) with face names as in ``FACES`` or None."""
         d = _lib.LbmDesc()
         d.q = self.Q
         d.storage = _lib.STORAGE_NAMES[self.storage]
@@ -123,6 +127,15 @@ class LBMKernel:
         d.block_x = self.block_x
         d.row_vec = self.row_vec
         d.flags = _lib.LBM_FLAG_XCD_REMAP if self.xcd_remap else 0
+        if faces is not None:
+            for slot, spec in zip(("inlet", "outlet"), faces):
+                if spec is None:
+                    continue
+                name, rho = spec
+                if name not in self.FACES or self.FACES.index(name) // 2 >= self.D:
+                    raise ValueError(f"unknown or out-of-lattice face '{name}'")
+                setattr(d, f"{slot}_face", self.FACES.index(name) + 1)
+                setattr(d, f"{slot}_rho", float(rho))
         return d
 
     @property
@@ -179,10 +192,13 @@ class LBMKernel:
 
     def step(self, f: torch.Tensor, nsteps: int = 1, *, mask=None, force=None,
              tau_field: Optional[torch.Tensor] = None, moments: bool = True,
-             out: Optional[torch.Tensor] = None, work: Optional[torch.Tensor] = None):
+             out: Optional[torch.Tensor] = None, work: Optional[torch.Tensor] = None,
+             faces=None):
         """Advance ``nsteps`` LBMNode.update steps (lbm.py:899-986).
 
-        Returns (f_new, rho, u, p); the moments are None when moments=False.
+        ``faces``: optional Zou-He pressure faces ((inlet, rho_in), (outlet,
+        rho_out)), either entry None (lbm.py:950-968). Returns (f_new, rho,
+        u, p); the moments are None when moments=False.
         """
         self._check_state(f)
         ff, fu = self._field(force)
@@ -198,7 +214,7 @@ class LBMKernel:
             out = self.empty()
         else:
             self._check_state(out)
-        d = self.desc(force=fu)
+        d = self.desc(force=fu, faces=faces)
         if work is None:
             work = self.workspace(nsteps)
         rho = u = p = None

@@ -201,12 +201,14 @@ class LBMNode(SimulationNode):
                 if force.shape != (self._D,):
                     raise ValueError(f"uniform body_force must have shape ({self._D},)")
                 force = [float(v) for v in force.tolist()]
-        if (boundary_inputs.get("inlet_pressure") is not None
-                or boundary_inputs.get("outlet_pressure") is not None):
-            raise NotImplementedError(
-                "Zou-He pressure faces (inlet_pressure/outlet_pressure) are not in the GPU "
-                "kernel yet")
-        return mask, force
+        # Zou-He faces: rho_p = P / cs2 at the configured faces (lbm.py:950-968)
+        faces = None
+        p_in = boundary_inputs.get("inlet_pressure")
+        p_out = boundary_inputs.get("outlet_pressure")
+        if p_in is not None or p_out is not None:
+            faces = (None if p_in is None else (self._inlet_face, float(p_in) / self._cs2),
+                     None if p_out is None else (self._outlet_face, float(p_out) / self._cs2))
+        return mask, force, faces
 
     def update(self, state: dict, boundary_inputs: dict, dt: float) -> dict:
         """One LBM step (lbm.py:899-986)."""
@@ -215,10 +217,10 @@ class LBMNode(SimulationNode):
     def run(self, state: dict, boundary_inputs: dict, nsteps: int) -> dict:
         """``nsteps`` updates with the same boundary inputs in one kernel
         sequence; equal to calling ``update`` nsteps times."""
-        mask, force = self._inputs(state, boundary_inputs)
+        mask, force, faces = self._inputs(state, boundary_inputs)
         k = self._kernel
         f_new, rho, u, p = k.step(self._soa(state["f"]), nsteps, mask=mask.to(torch.uint8),
-                                  force=force)
+                                  force=force, faces=faces)
         result = {
             "f": k.aos_view(f_new),
             "density": rho.to(self._dtype),

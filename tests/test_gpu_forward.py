@@ -30,7 +30,7 @@ def rel(a, b):
 
 def run_both(shape, lattice, tau, nsteps, *, rho0=None, u0=None, storage="f32",
              scheme="pull", bb="none", mask=None, force=None, tau_field=None,
-             lid=(0.0, 0.0, 0.0), f0_aos=None, steps_per_call=None, row_vec=0):
+             lid=(0.0, 0.0, 0.0), f0_aos=None, steps_per_call=None, row_vec=0, faces=None):
     """Same initial state through the oracle and the HIP kernel."""
     Q = 19 if lattice == "D3Q19" else 9
     e, w, _ = orc.lattice(Q)
@@ -52,13 +52,20 @@ def run_both(shape, lattice, tau, nsteps, *, rho0=None, u0=None, storage="f32",
     done = 0
     while done < nsteps:
         n = min(per, nsteps - done)
-        f, rho_g, u_g, p_g = k.step(f, n, mask=mask_t, force=force_arg, tau_field=tf)
+        f, rho_g, u_g, p_g = k.step(f, n, mask=mask_t, force=force_arg, tau_field=tf,
+                                    faces=faces)
         done += n
     torch.cuda.synchronize()
     fo = f0_aos.astype(np.float64)
     tau_o = tau if tau_field is None else tau_field.astype(np.float32).astype(np.float64)
+    zh = {}
+    if faces is not None:
+        if faces[0] is not None:
+            zh["inlet"] = (faces[0][0], faces[0][1] / 3.0)   # oracle takes pressure
+        if faces[1] is not None:
+            zh["outlet"] = (faces[1][0], faces[1][1] / 3.0)
     out = orc.run(fo, nsteps, tau=tau_o, force=force, mask=mask,
-                  bb="halfway" if bb == "halfway" else "reference", lid_velocity=lid)
+                  bb="halfway" if bb == "halfway" else "reference", lid_velocity=lid, **zh)
     gpu = {"f": k.to_aos(f, torch.float64).cpu().numpy(), "density": rho_g.cpu().numpy(),
            "velocity": u_g.cpu().numpy(), "pressure": p_g.cpu().numpy()}
     return gpu, out
@@ -240,3 +247,38 @@ def test_row_kernel_matches_oracle(lattice, shape, storage, vec):
                     scheme="pull", force=tuple([1e-5] * D), row_vec=vec)
     assert rel(g["f"], o["f"]) < (1e-12 if storage == "f64" else 1e-5)
     assert rel(g["velocity"], o["velocity"]) < 1e-5
+
+
+# --- Zou-He pressure faces (lbm.py:425-524) -----------------------------------
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_zou_he_faces_d2q9_channel_f64(scheme):
+    shape = (40, 12)
+    mask = np.zeros(shape, np.uint8)
+    mask[:, 0] = mask[:, -1] = 1
+    g, o = run_both(shape, "D2Q9", 0.8, 30, storage="f64", scheme=scheme, bb="reference",
+                    mask=mask, faces=(("x_min", 1.01), ("x_max", 1.0)))
+    assert rel(g["f"], o["f"]) < 1e-12
+    assert rel(g["velocity"], o["velocity"]) < 1e-6
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_zou_he_faces_d3q19_other_axes_f64(scheme):
+    """Inlet on y_min and outlet on z_max (two different axes, shared edge),
+    random walls: checks the axis mapping and the inlet-then-outlet order."""
+    shape = (9, 7, 8)
+    rng = np.random.default_rng(9)
+    mask = (rng.random(shape) < 0.1).astype(np.uint8)
+    f0 = orc.equilibrium(1 + 0.01 * rng.standard_normal(shape),
+                         0.02 * rng.standard_normal(shape + (3,)), *orc.lattice(19)[:2])
+    g, o = run_both(shape, "D3Q19", 0.9, 12, storage="f64", scheme=scheme, bb="reference",
+                    mask=mask, f0_aos=f0, faces=(("y_min", 1.02), ("z_max", 0.99)))
+    assert rel(g["f"], o["f"]) < 1e-12
+
+
+def test_zou_he_faces_f32_periodic_row_path_off():
+    """Faces on a periodic lattice whose rows could use the row kernel: the
+    faces must still be applied (the row kernel is bypassed)."""
+    shape = (6, 5, 64)
+    g, o = run_both(shape, "D3Q19", 0.7, 10, storage="f32", scheme="pull", row_vec=1,
+                    faces=(("z_min", 1.01), None))
+    assert rel(g["density"], o["density"]) < 1e-5

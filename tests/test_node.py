@@ -182,3 +182,37 @@ def test_node_matches_oracle_with_walls_and_force():
     # kernel tests (1.5e-7)
     err_u = np.max(np.abs(st["velocity"].double().cpu().numpy() - out["velocity"]))
     assert err_u < 1.5e-7, err_u
+
+
+@gpu
+def test_pressure_driven_poiseuille_2d():
+    """The reference's own test (tests/nodes/test_lbm.py:248-329) on the GPU node:
+    Zou-He inlet/outlet, 64x16 channel, 2000 steps, normalised profile within
+    atol 0.15 of the parabola."""
+    import torch
+    nx, ny = 64, 16
+    wall = np.zeros((nx, ny), dtype=bool)
+    wall[:, 0] = True
+    wall[:, ny - 1] = True
+    viscosity = 0.1
+    node = LBMNode(name="poiseuille", timestep=0.001, grid_shape=(nx, ny), viscosity=viscosity,
+                   lattice="D2Q9", wall_mask=wall, inlet_face="x_min", outlet_face="x_max")
+    st = node.initial_state()
+    bi = {"inlet_pressure": 1.0 / 3.0 * 1.01, "outlet_pressure": 1.0 / 3.0 * 1.00}
+    for _ in range(2000):
+        st = node.update(st, bi, node.delta_t)
+    assert torch.isfinite(st["velocity"]).all()
+    u = st["velocity"][nx // 2, :, 0].double().cpu().numpy()
+    np.testing.assert_allclose(u[0], 0.0, atol=1e-6)
+    np.testing.assert_allclose(u[ny - 1], 0.0, atol=1e-6)
+    inner = u[1:ny - 1]
+    assert np.all(inner > 0)
+    assert abs(int(np.argmax(inner)) + 1 - ny // 2) <= 2
+    H = ny - 2
+    y = np.arange(1, ny - 1) - 0.5
+    ana = y * (H - y)
+    np.testing.assert_allclose(inner / inner.max(), ana / ana.max(), atol=0.15)
+    # the same 2000 steps through one multi-step call agree with the updates
+    st2 = node.run(node.initial_state(), bi, 2000)
+    np.testing.assert_allclose(st2["velocity"].cpu().numpy(), st["velocity"].cpu().numpy(),
+                               atol=2e-6)
