@@ -15,7 +15,8 @@
  * (src/maddening/nodes/lbm.py:90-173).
  *
  * Storage types: LBM_F32/LBM_F64 hold f itself; LBM_F32_DEV, LBM_F16_DEV and
- * LBM_BF16_DEV hold the deviation f_q - w_q. Collision arithmetic is FP32
+ * LBM_BF16_DEV hold the deviation f_q - w_q (LBM_F16_DEV scaled by 2^10 so
+ * that small deviations stay out of the subnormal range). Collision arithmetic is FP32
  * for every storage type but F64, which computes in FP64.
  *
  * Auxiliary per-cell arrays use the reference's layouts (last grid axis
@@ -48,7 +49,7 @@ typedef enum lbm_status {
 typedef enum lbm_storage {
   LBM_F32 = 0,      /* float f */
   LBM_F64 = 1,      /* double f */
-  LBM_F16_DEV = 2,  /* IEEE half, f - w */
+  LBM_F16_DEV = 2,  /* IEEE half, (f - w) * 1024 */
   LBM_BF16_DEV = 3, /* bfloat16, f - w */
   LBM_F32_DEV = 4   /* float, f - w */
 } lbm_storage;

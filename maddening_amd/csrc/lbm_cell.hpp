@@ -193,7 +193,7 @@ __device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C
   if (a.rho || a.u || a.p) write_moments<L, C>(a, n.cell, self != 0, rho, ua, ub, uc);
   if (COLLIDE) {
     const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
-    collide<L, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+    collide_for<L, T, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
   }
   if (STORE) {
     T* dst = static_cast<T*>(a.dst);

@@ -93,15 +93,20 @@ template <> struct Storage<f32dev_t> {
   }
 };
 
+// IEEE half holds (f - w) * 2^10: deviations stay below ~1, so the scaled
+// values stay far from the 65504 limit while everything above 6e-8 (instead
+// of 6e-5) keeps a full 11-bit mantissa. Scaling by a power of two is exact.
+constexpr float kHalfScale = 1024.0f;
+
 template <> struct Storage<_Float16> {
   using C = float;
   template <class L>
   static __device__ __forceinline__ float load(const _Float16* p, int) {
-    return (float)(*p);
+    return (float)(*p) * (1.0f / kHalfScale);
   }
   template <class L>
   static __device__ __forceinline__ void store(_Float16* p, int, float d) {
-    __builtin_nontemporal_store((_Float16)d, p);
+    __builtin_nontemporal_store((_Float16)(d * kHalfScale), p);
   }
 };
 
@@ -152,14 +157,35 @@ __device__ __forceinline__ void moments(const C (&d)[L::Q], C Fa, C Fb, C Fc,
 //   f* = f - (f - feq)/tau + (1 - 1/(2 tau)) w [3 (e - u).F + 9 (e.u)(e.F)]
 //   feq - w = w [ (rho - 1) + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u.u) ]
 // cs2 = 1/3 gives the exact constants 3, 4.5, 1.5 and 9.
+//
+// Evaluated once per pair (q, opp q): e.u and e.F change sign between the
+// two, so feq splits into a shared even part A = (rho - 1) + rho (4.5 (e.u)^2
+// - 1.5 u.u) and an odd part 3 rho e.u, and the source into the even
+// 9 (e.u)(e.F) - 3 u.F and the odd 3 e.F. Multiply-adds are written as
+// explicit fma so every kernel variant rounds identically.
+//
+// FIXUP (float32 storage): in exact arithmetic the collision keeps
+// sum d = rho - 1 and moves sum d e from j to j + F. Float32 rounding misses
+// both by about an ulp per step, which accumulates in rho and u over a run,
+// so the miss is projected back onto the conserved moments: adding
+// w_q (a + 3 e_q . b) changes sum d by a and sum d e by b, because sum w = 1,
+// sum w e = 0 and sum w e e = I/3. (With 16-bit storage the rounding of the
+// stored values dominates and the fix-up buys nothing.)
 // ---------------------------------------------------------------------------
-template <class L, class C>
-__device__ __forceinline__ void collide(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
-                                        C Fc, C drho, C rho, C ua, C ub,
-                                        C uc) {
-  // raw first moment of the incoming populations (used by the float32 fix-up)
+template <class C>
+__device__ __forceinline__ C fmadd(C a, C b, C c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fmadd<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
+template <class L, class C, bool FIXUP>
+__device__ __forceinline__ void collide_impl(C (&d)[L::Q], C inv_tau, C Fa, C Fb, C Fc, C drho,
+                                             C rho, C ua, C ub, C uc) {
   C ja = 0, jb = 0, jc = 0;
-  if constexpr (sizeof(C) == 4) {
+  if constexpr (FIXUP) {
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
       if (L::ea[q] > 0) ja += d[q];
@@ -170,26 +196,37 @@ __device__ __forceinline__ void collide(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
       if (L::ec[q] < 0) jc -= d[q];
     }
   }
-  const C usq15 = C(1.5) * (ua * ua + ub * ub + uc * uc);
-  const C uF = ua * Fa + ub * Fb + uc * Fc;
+  const C usq15 = C(1.5) * fmadd(ua, ua, fmadd(ub, ub, uc * uc));
+  const C uF3 = C(3) * fmadd(ua, Fa, fmadd(ub, Fb, uc * Fc));
   const C keep = C(1) - inv_tau;
   const C src = C(1) - C(0.5) * inv_tau;
+  const C rho3 = C(3) * rho;
+  const C rho45 = C(4.5) * rho;
+  // rest population: e = 0
+  {
+    const C w = weight<L, C>(0);
+    const C A = fmadd(-rho, usq15, drho);
+    d[0] = fmadd(keep, d[0], fmadd(inv_tau * w, A, -(src * w) * uF3));
+  }
 #pragma unroll
-  for (int q = 0; q < L::Q; ++q) {
+  for (int q = 1; q < L::Q; ++q) {
+    if (L::opp[q] < q) continue;  // each pair once, from its first member
+    const int m = L::opp[q];
     const C w = weight<L, C>(q);
     const C eu = C(L::ea[q]) * ua + C(L::eb[q]) * ub + C(L::ec[q]) * uc;
     const C eF = C(L::ea[q]) * Fa + C(L::eb[q]) * Fb + C(L::ec[q]) * Fc;
-    const C deq = w * (drho + rho * (C(3) * eu + C(4.5) * eu * eu - usq15));
-    const C S = src * w * (C(3) * (eF - uF) + C(9) * eu * eF);
-    d[q] = keep * d[q] + inv_tau * deq + S;
+    const C A = fmadd(rho45 * eu, eu, fmadd(-rho, usq15, drho));  // even part of (feq - w)/w
+    const C B = rho3 * eu;                                         // odd part
+    const C sw = src * w;
+    const C wt = inv_tau * w;
+    const C even = fmadd(wt, A, sw * fmadd(C(9) * eu, eF, -uF3));
+    const C odd = fmadd(wt, B, (C(3) * sw) * eF);
+    const C dq = d[q];
+    const C dm = d[m];
+    d[q] = fmadd(keep, dq, even + odd);
+    d[m] = fmadd(keep, dm, even - odd);
   }
-  if constexpr (sizeof(C) == 4) {
-    // In exact arithmetic the collision keeps sum d = rho - 1 and moves
-    // sum d e from j to j + F. Float32 rounding misses both by about an ulp
-    // per step, and that error accumulates in rho and u over a run. Project
-    // the miss back onto the conserved moments: adding w_q (a + 3 e_q . b)
-    // changes sum d by a and sum d e by b, because sum w = 1, sum w e = 0
-    // and sum w e e = I/3.
+  if constexpr (FIXUP) {
     C m0 = 0, ma = 0, mb = 0, mc = 0;
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
@@ -208,8 +245,18 @@ __device__ __forceinline__ void collide(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
     const C bc = C(3) * ((jc + Fc) - mc);
 #pragma unroll
     for (int q = 0; q < L::Q; ++q)
-      d[q] += weight<L, C>(q) * (a0 + C(L::ea[q]) * ba + C(L::eb[q]) * bb + C(L::ec[q]) * bc);
+      d[q] = fmadd(weight<L, C>(q), a0 + C(L::ea[q]) * ba + C(L::eb[q]) * bb + C(L::ec[q]) * bc,
+                   d[q]);
   }
+}
+
+// Collision for storage T: the float32 fix-up where the stored values are
+// float32 (f or f - w), none for float64 (exact enough) or 16-bit storage.
+template <class L, class T, class C>
+__device__ __forceinline__ void collide_for(C (&d)[L::Q], C inv_tau, C Fa, C Fb, C Fc, C drho,
+                                            C rho, C ua, C ub, C uc) {
+  constexpr bool FIXUP = sizeof(C) == 4 && sizeof(T) == 4;
+  collide_impl<L, C, FIXUP>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
 }
 
 // ---------------------------------------------------------------------------

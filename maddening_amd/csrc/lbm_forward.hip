@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) k_push(FwdArgs<typename Storage<T>::C> a)
   C drho, rho, ua, ub, uc;
   moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
   const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
-  collide<L, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+  collide_for<L, T, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
     const int t = n.template at<L>(q, +1);
@@ -181,19 +181,20 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
 #pragma unroll
       for (int k = 0; k < V; ++k) d[q][k] = v[k];
     } else if (L::ec[q] > 0) {
-      // cell c takes c - 1: lane's first cell takes the previous lane's last
-      C edge = lane_from_prev(v[V - 1]);
-      if (segs > 1 && lane == 0) edge = St::template load<L>(rowp + c_before, q);
-      d[q][0] = edge;
+      // cell c takes c - 1: lane's first cell takes the previous lane's last;
+      // lane 0 takes the cell before the segment (a wave-uniform load, no branch)
+      const C before = St::template load<L>(rowp + c_before, q);
+      const C edge = lane_from_prev(v[V - 1]);
+      d[q][0] = lane == 0 ? before : edge;
 #pragma unroll
       for (int k = 1; k < V; ++k) d[q][k] = v[k - 1];
     } else {
       // cell c takes c + 1: lane's last cell takes the next lane's first
-      C edge = lane_from_next(v[0]);
-      if (segs > 1 && lane == 63) edge = St::template load<L>(rowp + c_after, q);
+      const C after = St::template load<L>(rowp + c_after, q);
+      const C edge = lane_from_next(v[0]);
 #pragma unroll
       for (int k = 0; k < V - 1; ++k) d[q][k] = v[k + 1];
-      d[q][V - 1] = edge;
+      d[q][V - 1] = lane == 63 ? after : edge;
     }
   }
 
@@ -212,12 +213,13 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
     if (a.rho || a.u || a.p) write_moments<L, C>(a, cell, false, rho, ua, ub, uc);
     if (COLLIDE) {
       const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[cell]) : a.inv_tau;
-      collide<L, C>(dk, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C>(dk, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
     }
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
       // same conversion as Storage<T>::store, kept in registers for a vector store
       if constexpr (sizeof(T) >= 4) out[q][k] = T(weight<L, C>(q) + dk[q]);
+      else if constexpr (std::is_same<T, _Float16>::value) out[q][k] = T(dk[q] * kHalfScale);
       else out[q][k] = T(dk[q]);
     }
   }
@@ -417,12 +419,19 @@ FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
 
 
 // Vector width of the row-per-wave kernel for this lattice (0: not eligible).
-// On MI355X the one-cell-per-thread kernel measured faster than the
-// row-per-wave kernel at every width (256^3 D3Q19: 5.6 vs 5.0 TB/s f32,
-// 5.4 vs 4.0 TB/s f16), so auto (row_vec 0) keeps it off.
+// Auto (row_vec 0), from 256^3 D3Q19 measurements on MI355X: 16-bit
+// storage uses the row kernel with 2 cells per lane (4-byte lane accesses:
+// 5.85 TB/s against 5.27 TB/s for the one-cell-per-thread kernel); 32-bit
+// storage keeps the one-cell-per-thread kernel (5.69 TB/s, the same as the
+// best row variant within noise).
 inline int row_width(const lbm_desc* d, const Geom& g) {
-  if (d->bb_mode != LBM_BB_NONE || d->row_vec <= 0) return 0;
+  if (d->bb_mode != LBM_BB_NONE || d->row_vec < 0) return 0;
   if (d->inlet_face > 0 || d->outlet_face > 0) return 0;  // faces live in the per-cell kernels
+  if (d->row_vec == 0) {
+    if (storage_size(d->storage) != 2) return 0;
+    if (g.nc % 128 == 0) return 2;
+    return g.nc % 64 == 0 ? 1 : 0;
+  }
   return g.nc % (64 * d->row_vec) == 0 ? d->row_vec : 0;
 }
 
