@@ -183,6 +183,32 @@ def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=4):
                "adjoint_over_forward": round(e1.elapsed_time(e2) / nsteps / fwd_ms, 3),
                "fwd_plus_adjoint_ms_per_step": round(e0.elapsed_time(e2) / nsteps, 4),
                "loss": float(L), "dL_dF": float(Fg.grad[2])}
+        if storage == "f32":
+            # every post-collision state kept (N x 319 MB): no recompute
+            del fN
+            torch.cuda.empty_cache()
+            Fa = torch.tensor([0.0, 0.0, F0], dtype=torch.float64, device=dev,
+                              requires_grad=True)
+            ta = tau.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            e0.record()
+            fA, _, _, _ = rollout(k, f0, nsteps, force=Fa, tau_field=ta, mask=mask,
+                                  checkpoint="all")
+            e1.record()
+            uz_loss(k, fA, Fa[2]).backward()
+            e2.record()
+            torch.cuda.synchronize()
+            rec["all_states"] = {
+                "adjoint_ms_per_step": round(e1.elapsed_time(e2) / nsteps, 4),
+                "adjoint_over_forward": round(e1.elapsed_time(e2) / nsteps / fwd_ms, 3),
+                "fwd_plus_adjoint_ms_per_step": round(e0.elapsed_time(e2) / nsteps, 4),
+                "states_GB": round(nsteps * k.ncells * 19 * 4 / 1e9, 1),
+                "dL_dF": float(Fa.grad[2]),
+                "max_rel_diff_dL_dtau_vs_sqrt": float((ta.grad - tg.grad).abs().max()
+                                                      / tg.grad.abs().max())}
+            del fA, Fa, ta
+            fN = None
+            torch.cuda.empty_cache()
         if storage == "f64":
             def loss_at(Fz, tf):
                 f, _, _, _ = k.step(f0, nsteps, mask=mask, force=[0, 0, Fz], tau_field=tf,

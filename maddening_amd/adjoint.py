@@ -7,8 +7,9 @@ discrete adjoint: the moments' cotangents are pulled back onto f_N
 the last to the first. States the backward needs are recomputed from
 checkpoints: with ``checkpoint="sqrt"`` every K = ceil(sqrt(N))-th state is
 kept during the forward pass and each K-step segment is recomputed once
-(N/K + K states resident); ``checkpoint="all"`` keeps every state and
-recomputes nothing; an int sets K.
+(N/K + K states resident, about N extra launches); ``checkpoint="all"``
+keeps every state (N states resident) and recomputes nothing; an int sets K.
+The forward pass itself is exactly lbm_step's N + 1 launches.
 
 Gradients flow to f0, to a uniform body force (tensor of shape (D,)) or a
 force field (*G, D), and to a per-cell relaxation-time field (*G).
@@ -36,37 +37,69 @@ def segment_length(nsteps: int, checkpoint: Union[str, int]) -> int:
     return k
 
 
+def _pull(L, st, d, phase, src, dst, m, ff, tf, rho=None, u=None, p=None):
+    _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), phase, _ptr(src), _ptr(dst), _ptr(m),
+                                _ptr(ff), _ptr(tf), _ptr(rho), _ptr(u), _ptr(p)),
+               "lbm_pull_phase")
+
+
 class _Rollout(torch.autograd.Function):
+    """The forward pass runs the pull sequence of lbm_step launch by launch:
+    g_0 = C(f_0), g_k = C(S B g_{k-1}), f_N = S B g_{N-1}. The starting state
+    of step k is f_0 (k = 0) or is gathered from g_{k-1}. A segment starting
+    at step s is therefore checkpointed by keeping f_0 (s = 0) or g_{s-1}."""
+
     @staticmethod
     def forward(ctx, f0, force, tau_field, kern: LBMKernel, nsteps: int, mask, seg: int):
         ctx.set_materialize_grads(False)
+        L = _lib.lib()
+        dev = kern.device
+        st = _stream(dev)
         # a uniform force goes to the kernels as host numbers: read it once
         fk = force.tolist() if (force is not None and force.dim() == 1) else force
-        ckpts = []
-        f = f0
-        done = 0
-        while done < nsteps:
-            n = min(seg, nsteps - done)
-            ckpts.append(f)
-            f, _, _, _ = kern.step(f, n, mask=mask, force=fk, tau_field=tau_field,
-                                   moments=False)
-            done += n
-        rho, u, p = kern.moments(f, mask=mask, force=fk)
+        ff, fu = kern._field(fk)
+        d = kern.desc(force=fu)
+        m = kern._mask(mask)
+        if kern.bb != "none" and m is None:
+            raise ValueError(f"bb='{kern.bb}' needs a wall mask")
+        tf = None if tau_field is None else tau_field.to(dev, torch.float32).contiguous()
+        ckpts = [f0]           # source of each segment's first state
+        # the last segment's states come straight from this pass: no recompute
+        s_last = ((nsteps - 1) // seg) * seg
+        tail = []
+        pool = [kern.empty(), kern.empty()]
+        prev = f0
+        for k in range(nsteps):
+            ckpt = (k + 1) % seg == 0 and k + 1 < nsteps   # g_k starts segment k + 1
+            in_tail = s_last <= k <= nsteps - 2
+            g = kern.empty() if (ckpt or in_tail) else pool[k & 1]
+            _pull(L, st, d, 0 if k == 0 else 1, prev, g, m, ff, tf)
+            if ckpt:
+                ckpts.append(g)
+            if in_tail:
+                tail.append(g)
+            prev = g
+        fN = kern.empty()
+        rho = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
+        u = torch.empty(kern.grid_shape + (kern.D,), dtype=torch.float32, device=dev)
+        p = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
+        _pull(L, st, d, 2, prev, fN, m, ff, tf, rho, u, p)
         ctx.kern = kern
         ctx.nsteps = nsteps
         ctx.seg = seg
         ctx.mask = mask
         ctx.force_is_field = force is not None and force.dim() > 1
-        ctx.force_shape = None if force is None else tuple(force.shape)
         ctx.force_dtype = None if force is None else force.dtype
         ctx.force_list = fk if isinstance(fk, list) else None
-        ctx.save_for_backward(f, force, tau_field, *ckpts)
-        return f, rho, u, p
+        ctx.n_ckpt = len(ckpts)
+        ctx.save_for_backward(fN, force, tau_field, *ckpts, *tail)
+        return fN, rho, u, p
 
     @staticmethod
     def backward(ctx, g_f, g_rho, g_u, g_p):
         kern: LBMKernel = ctx.kern
-        f_N, force, tau_field, *ckpts = ctx.saved_tensors
+        f_N, force, tau_field, *rest = ctx.saved_tensors
+        ckpts, tail = rest[:ctx.n_ckpt], rest[ctx.n_ckpt:]
         L = _lib.lib()
         dev = kern.device
         st = _stream(dev)
@@ -76,8 +109,6 @@ class _Rollout(torch.autograd.Function):
             ff, fu = kern._field(force)
         d = kern.desc(force=fu)
         m = kern._mask(ctx.mask)
-        if kern.bb != "none" and m is None:
-            raise ValueError("bb mode needs a wall mask")
         tf = None if tau_field is None else tau_field.to(dev, torch.float32).contiguous()
         need_f0, need_force, need_tau = ctx.needs_input_grad[0], ctx.needs_input_grad[1], \
             ctx.needs_input_grad[2]
@@ -98,23 +129,23 @@ class _Rollout(torch.autograd.Function):
                                          _ptr(c(g_rho)), _ptr(c(g_u)), _ptr(c(g_p)), _ptr(gin),
                                          _ptr(fbar), _ptr(fbar_field), _ptr(work)),
                    "lbm_moments_adjoint")
-        # Recompute each segment with the forward's pull launches, keeping the
-        # post-collision states g_0 .. g_{n-2}; step k's starting state is the
-        # checkpoint for k = 0 and is gathered from g_{k-1} by the adjoint
-        # kernel otherwise.
+        # Per segment (last first): recompute g_s .. g_{s+n-2} from the
+        # checkpoint with the forward's pull launches, then transpose steps
+        # s+n-1 .. s; step k's starting state is f_0 or gathered from g_{k-1}.
         spare = torch.empty_like(fbar)
-        nseg = len(ckpts)
-        for s in reversed(range(nseg)):
-            n = min(ctx.seg, ctx.nsteps - s * ctx.seg)
-            states = [ckpts[s]]
-            for k in range(n - 1):
-                g = torch.empty(kern.state_shape, dtype=kern.dtype, device=dev)
-                _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 0 if k == 0 else 1,
-                                            _ptr(states[-1]), _ptr(g), _ptr(m), _ptr(ff),
-                                            _ptr(tf), None, None, None), "lbm_pull_phase")
+        for si in reversed(range(len(ckpts))):
+            s = si * ctx.seg
+            n = min(ctx.seg, ctx.nsteps - s)
+            states = [ckpts[si]]
+            if si == len(ckpts) - 1:
+                states += list(tail)
+            for i in range(len(states), n):
+                g = kern.empty()
+                _pull(L, st, d, 1 if (s > 0 or i > 1) else 0, states[-1], g, m, ff, tf)
                 states.append(g)
             for i in reversed(range(n)):
-                _lib.check(L.lbm_adjoint_step(st, ctypes.byref(d), _ptr(states[i]), int(i > 0),
+                post = int(s > 0 or i > 0)
+                _lib.check(L.lbm_adjoint_step(st, ctypes.byref(d), _ptr(states[i]), post,
                                               _ptr(fbar), _ptr(spare), _ptr(m), _ptr(ff),
                                               _ptr(tf), _ptr(tau_bar), _ptr(fbar_field),
                                               _ptr(work)), "lbm_adjoint_step")

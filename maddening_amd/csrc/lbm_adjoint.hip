@@ -148,27 +148,46 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       const C c = C(1) - C(0.5) * om;
       const C usq = ua * ua + ub * ub + uc * uc;
       const C uF = ua * Fa + ub * Fb + uc * Fc;
-      // sums over q of g_bar_q w_q times {1, e.u, (e.u)^2, e_a, (e.u) e_a, (e.F) e_a}
+      // Sums over q of gw_q = g_bar_q w_q times {1, e.u, (e.u)^2, e_a, (e.u) e_a,
+      // (e.F) e_a}, plus tb = sum_q g_bar_q d f*_q / d omega with
+      //   d f*_q / d omega = (feq_q - f_q) - 0.5 w [3 (e.F - u.F) + 9 (e.u)(e.F)].
+      // Taken per pair (q, opp q): e.u, e.F and e flip sign between the two,
+      // so only the sum s = gw_q + gw_m or the difference t = gw_q - gw_m of
+      // the pair enters each term.
       C G0 = 0, Geu = 0, Geu2 = 0, Ga = 0, Gb = 0, Gc = 0;
-      C Ha = 0, Hb = 0, Hc = 0;  // (3 om rho e.u + 9/... ) collected below
-      C Ka = 0, Kb = 0, Kc = 0;  // g w (e.u) e
-      C tb = 0;                   // sum g_bar (dC/domega) for tau
+      C Ha = 0, Hb = 0, Hc = 0;  // sum gw (e.F) e
+      C Ka = 0, Kb = 0, Kc = 0;  // sum gw (e.u) e
+      C tb = 0;
+      const C Arest = fmadd(-rho, C(1.5) * usq, drho);  // even part of (feq - w)/w at e.u = 0
+      const C uF3 = C(3) * uF;
+      {
+        const C w = weight<L, C>(0);
+        const C gw = gb[0] * w;
+        G0 = gw;
+        // rest population: (feq - w)/w = Arest, source -3 u.F
+        tb = fmadd(gw, fmadd(C(0.5), uF3, Arest), -gb[0] * d[0]);
+      }
 #pragma unroll
-      for (int q = 0; q < L::Q; ++q) {
+      for (int q = 1; q < L::Q; ++q) {
+        if (L::opp[q] < q) continue;
+        const int m = L::opp[q];
         const C w = weight<L, C>(q);
-        const C gw = gb[q] * w;
+        const C gq = gb[q] * w, gm = gb[m] * w;
+        const C sm = gq + gm, df = gq - gm;
         const C eu = C(L::ea[q]) * ua + C(L::eb[q]) * ub + C(L::ec[q]) * uc;
         const C eF = C(L::ea[q]) * Fa + C(L::eb[q]) * Fb + C(L::ec[q]) * Fc;
-        G0 += gw;
-        Geu += gw * eu;
-        Geu2 += gw * eu * eu;
-        if (L::ea[q]) { Ga += L::ea[q] * gw; Ka += L::ea[q] * gw * eu; Ha += L::ea[q] * gw * eF; }
-        if (L::eb[q]) { Gb += L::eb[q] * gw; Kb += L::eb[q] * gw * eu; Hb += L::eb[q] * gw * eF; }
-        if (L::ec[q]) { Gc += L::ec[q] * gw; Kc += L::ec[q] * gw * eu; Hc += L::ec[q] * gw * eF; }
-        // d f*_q / d omega = (feq_q - f_q) - 0.5 w [3 (e.F - u.F) + 9 (e.u)(e.F)]
-        const C deq = w * (drho + rho * (C(3) * eu + C(4.5) * eu * eu - C(1.5) * usq));
-        const C sraw = w * (C(3) * (eF - uF) + C(9) * eu * eF);
-        tb += gb[q] * (deq - d[q] - C(0.5) * sraw);
+        G0 += sm;
+        Geu = fmadd(df, eu, Geu);
+        Geu2 = fmadd(sm * eu, eu, Geu2);
+        const C seu = sm * eu, seF = sm * eF;
+        if (L::ea[q]) { Ga += C(L::ea[q]) * df; Ka += C(L::ea[q]) * seu; Ha += C(L::ea[q]) * seF; }
+        if (L::eb[q]) { Gb += C(L::eb[q]) * df; Kb += C(L::eb[q]) * seu; Hb += C(L::eb[q]) * seF; }
+        if (L::ec[q]) { Gc += C(L::ec[q]) * df; Kc += C(L::ec[q]) * seu; Hc += C(L::ec[q]) * seF; }
+        // even part: A - 0.5 E with A = Arest + 4.5 rho (e.u)^2, E = 9 (e.u)(e.F) - 3 u.F;
+        // odd part:  B - 0.5 O with B = 3 rho e.u, O = 3 e.F
+        const C even = fmadd(C(4.5) * rho * eu, eu, Arest) - C(0.5) * fmadd(C(9) * eu, eF, -uF3);
+        const C odd = fmadd(C(3) * rho, eu, C(-1.5) * eF);
+        tb = fmadd(sm, even, fmadd(df, odd, tb)) - fmadd(gb[q], d[q], gb[m] * d[m]);
       }
       // cotangents of rho (u held fixed) and u from feq (times omega) and the source (times c)
       const C rho_bar = om * (G0 * (C(1) - C(1.5) * usq) + C(3) * Geu + C(4.5) * Geu2);
