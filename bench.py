@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -25,6 +26,27 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 ROOT = os.path.dirname(os.path.abspath(__file__))
+STORAGE_BYTES = {"f32": 4, "f32dev": 4, "f64": 8, "f16dev": 2, "bf16dev": 2}
+
+
+def fused_kernel_label(storage, row_vec):
+    """Name of the kernel that runs the fused pull+collide launches."""
+    if row_vec > 0:
+        return f"k_row<D3Q19,{storage},V={row_vec}>"
+    return f"k_cell<D3Q19,{storage},pull+collide>"
+
+
+def auto_row_vec(storage, nc, row_vec):
+    """Row width the library picks for a periodic pull run (lbm_forward.hip
+    row_width): 2 cells per lane for 16-bit storage when rows split into
+    128-cell wave segments, the per-cell kernel otherwise."""
+    if row_vec < 0:
+        return 0
+    if row_vec == 0:
+        if STORAGE_BYTES[storage] != 2:
+            return 0
+        return 2 if nc % 128 == 0 else (1 if nc % 64 == 0 else 0)
+    return row_vec if nc % (64 * row_vec) == 0 else 0
 
 
 def taylor_green_device(n, u0, device):
@@ -123,7 +145,7 @@ def precision_leg(n, dev, checkpoints=(100, 1000, 5000)):
     return out
 
 
-def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=4):
+def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=4, reps=3):
     """Sphere-packed channel (porosity 0.7, radii 6-12, seed 0), tau field 0.7,
     body force (0, 0, 1e-5), loss = mean u_z over the outlet plane. Times the
     float32 forward and the sqrt(N)-checkpointed adjoint; checks the float64
@@ -150,38 +172,49 @@ def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=4):
         k = LBMKernel(shape, tau=0.7, storage=storage, bb="reference", device=dev)
         f0 = k.equilibrium()
         tau = torch.full(shape, 0.7, device=dev)
-        # forward alone
+        # forward alone and forward + adjoint through the autograd rollout,
+        # interleaved `reps` times (medians reported): clocks drift by several
+        # percent between runs on a loaded box. One untimed rollout first so
+        # the checkpoint and segment buffers come from torch's cache.
         work = k.workspace(nsteps)
         k.step(f0, 10, mask=mask, force=[0, 0, F0], tau_field=tau, work=work)
-        torch.cuda.synchronize()
-        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        e0.record()
-        k.step(f0, nsteps, mask=mask, force=[0, 0, F0], tau_field=tau, work=work)
-        e1.record()
-        torch.cuda.synchronize()
-        fwd_ms = e0.elapsed_time(e1) / nsteps
-        # forward + adjoint through the autograd rollout; one untimed pass first
-        # so the checkpoint and segment buffers come from torch's cache
         Fw = torch.tensor([0.0, 0.0, F0], dtype=torch.float64, device=dev, requires_grad=True)
         tw = tau.clone().requires_grad_(True)
         fw, _, _, _ = rollout(k, f0, nsteps, force=Fw, tau_field=tw, mask=mask)
         uz_loss(k, fw, Fw[2]).backward()
         del fw, Fw, tw
-        Fg = torch.tensor([0.0, 0.0, F0], dtype=torch.float64, device=dev, requires_grad=True)
-        tg = tau.clone().requires_grad_(True)
         torch.cuda.synchronize()
-        e0.record()
-        fN, _, _, _ = rollout(k, f0, nsteps, force=Fg, tau_field=tg, mask=mask)
-        e1.record()
-        L = uz_loss(k, fN, Fg[2])
-        L.backward()
-        e2.record()
-        torch.cuda.synchronize()
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        fwd_t, roll_t, adj_t, tot_t = [], [], [], []
+        for _ in range(reps):
+            e0.record()
+            k.step(f0, nsteps, mask=mask, force=[0, 0, F0], tau_field=tau, work=work)
+            e1.record()
+            torch.cuda.synchronize()
+            fwd_t.append(e0.elapsed_time(e1) / nsteps)
+            Fg = torch.tensor([0.0, 0.0, F0], dtype=torch.float64, device=dev,
+                              requires_grad=True)
+            tg = tau.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            e0.record()
+            fN, _, _, _ = rollout(k, f0, nsteps, force=Fg, tau_field=tg, mask=mask)
+            e1.record()
+            L = uz_loss(k, fN, Fg[2])
+            L.backward()
+            e2.record()
+            torch.cuda.synchronize()
+            roll_t.append(e0.elapsed_time(e1) / nsteps)
+            adj_t.append(e1.elapsed_time(e2) / nsteps)
+            tot_t.append(e0.elapsed_time(e2) / nsteps)
+        fwd_ms, adj_ms = statistics.median(fwd_t), statistics.median(adj_t)
         rec = {"forward_ms_per_step": round(fwd_ms, 4),
-               "rollout_forward_ms_per_step": round(e0.elapsed_time(e1) / nsteps, 4),
-               "adjoint_ms_per_step": round(e1.elapsed_time(e2) / nsteps, 4),
-               "adjoint_over_forward": round(e1.elapsed_time(e2) / nsteps / fwd_ms, 3),
-               "fwd_plus_adjoint_ms_per_step": round(e0.elapsed_time(e2) / nsteps, 4),
+               "rollout_forward_ms_per_step": round(statistics.median(roll_t), 4),
+               "adjoint_ms_per_step": round(adj_ms, 4),
+               "adjoint_over_forward": round(adj_ms / fwd_ms, 3),
+               "fwd_plus_adjoint_ms_per_step": round(statistics.median(tot_t), 4),
+               "reps": reps,
+               "forward_ms_range": [round(min(fwd_t), 4), round(max(fwd_t), 4)],
+               "adjoint_ms_range": [round(min(adj_t), 4), round(max(adj_t), 4)],
                "loss": float(L), "dL_dF": float(Fg.grad[2])}
         if storage == "f32":
             # every post-collision state kept (N x 319 MB): no recompute
@@ -363,7 +396,8 @@ def main():
     f0 = kern.equilibrium(rho, u)
     ev, wall, launches, out = time_kernel(kern, f0, args.steps, args.warmup)
     mlups = cells * args.steps / ev / 1e6
-    b_cell = 2 * 19 * 4
+    b_cell = 2 * 19 * STORAGE_BYTES[args.storage]
+    rv = auto_row_vec(args.storage, n, args.row_vec) if kern.scheme != "push" else 0
     per_launch_s = ev / launches
     achieved = b_cell * cells / per_launch_s / 1e9
     result = {
@@ -380,12 +414,13 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"D3Q19 BGK periodic {n}^3 Taylor-Green vortex, u0=0.05, "
-                               f"tau=0.8, float32 populations ({args.storage})",
+                               f"tau=0.8, populations stored as {args.storage}",
                    "cells": cells, "scheme": kern.scheme, "wall_s_timed_call": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(args.storage),
-                     "kernel": f"k_cell<D3Q19,{args.storage},pull+collide> (152 B/cell algorithmic)",
+                     "kernel": f"{fused_kernel_label(args.storage, rv)} "
+                               f"({b_cell} B/cell algorithmic)",
                      "avg_launch_us": round(per_launch_s * 1e6, 2)},
     }
     extra = {}
@@ -393,11 +428,17 @@ def main():
         k2 = make(storage)
         f2 = k2.equilibrium(rho, u)
         ev2, _, launches2, _ = time_kernel(k2, f2, args.steps, args.warmup)
-        bc2 = 2 * 19 * (2 if storage in ("f16dev", "bf16dev") else 4)
+        bc2 = 2 * 19 * STORAGE_BYTES[storage]
         extra[storage] = {"MLUPS": round(cells * args.steps / ev2 / 1e6, 1),
                           "GB/s": round(bc2 * cells / (ev2 / launches2) / 1e9, 1),
-                          "speedup_vs_f32": round(ev / ev2, 3)}
+                          "speedup_vs_headline": round(ev / ev2, 3), "_ev": ev2}
         del f2, k2
+    ref32 = extra["f32"]["_ev"] if "f32" in extra else (ev if args.storage == "f32" else None)
+    for v in extra.values():
+        if ref32 is not None:
+            v["speedup_vs_f32"] = round(ref32 / v.pop("_ev"), 3)
+        else:
+            v.pop("_ev")
     if extra:
         result["variants"] = extra
     del out, f0

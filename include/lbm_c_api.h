@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LBM_API_VERSION 2
+#define LBM_API_VERSION 3
 
 typedef enum lbm_status {
   LBM_OK = 0,
@@ -93,6 +93,14 @@ typedef struct lbm_desc {
 
 /* flags */
 #define LBM_FLAG_XCD_REMAP 1  /* deal consecutive rows to one XCD (off by default: measured slower) */
+/* Adjoint cotangent convention under LBM_BB_REFERENCE (ignored for
+ * LBM_BB_NONE, rejected for LBM_BB_HALFWAY): the cotangent buffer holds the
+ * slots q and opp(q) of every wall cell exchanged. In that form the transpose
+ * of the node-swap bounce-back is a plain stencil gather, so a chain of
+ * lbm_adjoint_step calls reads no neighbour mask. IN applies to fbar_next /
+ * fbar_in, OUT to fbar_out. */
+#define LBM_FLAG_ADJ_IN_SWAPPED 2
+#define LBM_FLAG_ADJ_OUT_SWAPPED 4
 
 /* ---- queries ----------------------------------------------------------- */
 

@@ -18,6 +18,8 @@ LBM_F32, LBM_F64, LBM_F16_DEV, LBM_BF16_DEV, LBM_F32_DEV = 0, 1, 2, 3, 4
 LBM_BB_NONE, LBM_BB_REFERENCE, LBM_BB_HALFWAY = 0, 1, 2
 LBM_SCHEME_AUTO, LBM_SCHEME_PULL, LBM_SCHEME_PUSH = 0, 1, 2
 LBM_FLAG_XCD_REMAP = 1
+LBM_FLAG_ADJ_IN_SWAPPED = 2
+LBM_FLAG_ADJ_OUT_SWAPPED = 4
 
 STORAGE_NAMES = {"f32": LBM_F32, "f64": LBM_F64, "f16dev": LBM_F16_DEV,
                  "bf16dev": LBM_BF16_DEV, "f32dev": LBM_F32_DEV}

@@ -123,9 +123,21 @@ class _Rollout(torch.autograd.Function):
         def c(t):
             return None if t is None else t.to(dev, torch.float32).contiguous()
 
+        # Under the reference wall rule the cotangents between steps are kept
+        # with the slots of wall cells exchanged, so each transposed step is a
+        # plain gather (no neighbour mask reads); the last step writes dL/df_0
+        # in the ordinary form.
+        def with_flags(bits):
+            dd = _lib.LbmDesc.from_buffer_copy(d)
+            if kern.bb == "reference":
+                dd.flags |= bits
+            return dd
+        d_mom = with_flags(_lib.LBM_FLAG_ADJ_OUT_SWAPPED)
+        d_mid = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED | _lib.LBM_FLAG_ADJ_OUT_SWAPPED)
+        d_first = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED)
         fbar = torch.empty(kern.state_shape, dtype=kern.dtype, device=dev)
         gin = None if g_f is None else g_f.to(dev, kern.dtype).contiguous()
-        _lib.check(L.lbm_moments_adjoint(st, ctypes.byref(d), _ptr(f_N), _ptr(m), _ptr(ff),
+        _lib.check(L.lbm_moments_adjoint(st, ctypes.byref(d_mom), _ptr(f_N), _ptr(m), _ptr(ff),
                                          _ptr(c(g_rho)), _ptr(c(g_u)), _ptr(c(g_p)), _ptr(gin),
                                          _ptr(fbar), _ptr(fbar_field), _ptr(work)),
                    "lbm_moments_adjoint")
@@ -145,7 +157,8 @@ class _Rollout(torch.autograd.Function):
                 states.append(g)
             for i in reversed(range(n)):
                 post = int(s > 0 or i > 0)
-                _lib.check(L.lbm_adjoint_step(st, ctypes.byref(d), _ptr(states[i]), post,
+                dk = d_mid if post else d_first
+                _lib.check(L.lbm_adjoint_step(st, ctypes.byref(dk), _ptr(states[i]), post,
                                               _ptr(fbar), _ptr(spare), _ptr(m), _ptr(ff),
                                               _ptr(tf), _ptr(tau_bar), _ptr(fbar_field),
                                               _ptr(work)), "lbm_adjoint_step")

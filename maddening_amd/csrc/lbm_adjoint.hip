@@ -38,6 +38,8 @@ struct AdjArgs {
   long long plane;
   int na, nb, nc;
   int bpr, nblocks, remap;
+  int out_swapped;      // write fbar_out with the slots of wall cells exchanged
+  int in_swapped;       // fbar_in of the moments adjoint is in that form
   C inv_tau;
   C Fa, Fb, Fc;
   C lid_a, lid_b, lid_c;
@@ -99,7 +101,11 @@ __device__ __forceinline__ void read_force(const AdjArgs<C>& a, int cell, C& Fa,
 // FPOST: `f` holds the post-collision state of the previous step and the
 // step's starting state f_n is gathered from it exactly as the forward pull
 // does; otherwise `f` holds f_n itself.
-template <class L, class T, int BB, bool FPOST>
+// GSW (reference wall rule only): gin holds the slots q, opp(q) of every wall
+// cell exchanged, so the transposed node swap folds into a plain gather and
+// no neighbour mask is read. a.out_swapped writes gout in the same form,
+// with this cell's own mask.
+template <class L, class T, int BB, bool FPOST, bool GSW>
 __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C> a) {
   using C = typename Storage<T>::C;
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
@@ -127,7 +133,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
 #pragma unroll
       for (int p = 0; p < L::Q; ++p) {
         const int x = n.template at<L>(p, +1);
-        if (BB == BB_NONE) {
+        if (BB == BB_NONE || (BB == BB_REF && GSW)) {
           gb[p] = a.gin[p * P + x];
         } else if (BB == BB_REF) {
           gb[p] = a.gin[(a.mask[x] ? L::opp[p] : p) * P + x];
@@ -200,9 +206,14 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       const C Ba = ubar_a * inv_rho, Bb = ubar_b * inv_rho, Bc = ubar_c * inv_rho;
       const C A = rho_bar - (Ba * ua + Bb * ub + Bc * uc);
       const C keep = C(1) - om;
+      // exchanged form: slot p carries the cotangent of opp(p), whose e is -e_p
+      const bool xs = BB == BB_REF && a.out_swapped && self;
 #pragma unroll
-      for (int p = 0; p < L::Q; ++p)
-        a.gout[p * P + n.cell] = keep * gb[p] + A + C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
+      for (int p = 0; p < L::Q; ++p) {
+        const C eB = C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
+        const C g = xs ? gb[L::opp[p]] : gb[p];
+        a.gout[p * P + n.cell] = keep * g + A + (xs ? -eB : eB);
+      }
       if (a.tau_bar) a.tau_bar[n.cell] += float(-om * om * tb);
       // dL/dF: the source directly (u fixed) plus through u = (j + F/2)/rho
       dFa = c * (C(3) * Ga - C(3) * ua * G0 + C(9) * Ka) + C(0.5) * Ba;
@@ -251,10 +262,14 @@ __global__ void __launch_bounds__(256) k_moments_adjoint(AdjArgs<typename Storag
     const C inv_rho = C(1) / (rho > C(1e-10) ? rho : C(1e-10));
     const C Ba = va * inv_rho, Bb = vb * inv_rho, Bc = vc * inv_rho;
     const C A = rb - (Ba * ua + Bb * ub + Bc * uc);
+    // at a wall cell u_bar is dropped (B = 0), so exchanging slots touches
+    // only the incoming cotangent: read from opp(p) when exactly one of the
+    // two buffers is in the exchanged form
+    const bool cross = wall && (a.in_swapped != a.out_swapped);
 #pragma unroll
     for (int p = 0; p < L::Q; ++p) {
-      const C base = a.gin ? a.gin[p * P + cell] : C(0);
-      a.gout[p * P + cell] = base + A + C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
+      const C base = a.gin ? a.gin[(cross ? L::opp[p] : p) * P + cell] : C(0);
+      a.gout[p * P + cell] = base + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
     }
     dFa = C(0.5) * Ba;
     dFb = C(0.5) * Bb;
@@ -309,6 +324,9 @@ inline int adj_geom(const lbm_desc* d, Geom* g) {
     return fail(LBM_ERR_UNSUPPORTED, "the adjoint needs f32, f32dev or f64 storage");
   if (d->inlet_face > 0 || d->outlet_face > 0)
     return fail(LBM_ERR_UNSUPPORTED, "the adjoint does not cover Zou-He pressure faces");
+  if (d->bb_mode == LBM_BB_HALFWAY && (d->flags & (LBM_FLAG_ADJ_IN_SWAPPED | LBM_FLAG_ADJ_OUT_SWAPPED)))
+    return fail(LBM_ERR_UNSUPPORTED,
+                "exchanged-slot cotangents are defined for the reference wall rule only");
   g->bx = 256;
   g->bpr = (g->nc + 255) / 256;
   g->nblocks = g->na * g->nb * g->bpr;
@@ -328,6 +346,9 @@ AdjArgs<C> adj_args(const lbm_desc* d, const Geom& g) {
   a.bpr = g.bpr;
   a.nblocks = g.nblocks;
   a.remap = (d->flags & LBM_FLAG_XCD_REMAP) ? 1 : 0;
+  // the exchanged form only means something where walls swap slots
+  a.in_swapped = (d->bb_mode == LBM_BB_REFERENCE && (d->flags & LBM_FLAG_ADJ_IN_SWAPPED)) ? 1 : 0;
+  a.out_swapped = (d->bb_mode == LBM_BB_REFERENCE && (d->flags & LBM_FLAG_ADJ_OUT_SWAPPED)) ? 1 : 0;
   a.inv_tau = C(1.0 / d->tau);
   if (d->q == 19) {
     a.Fa = C(d->force[0]); a.Fb = C(d->force[1]); a.Fc = C(d->force[2]);
@@ -340,16 +361,20 @@ AdjArgs<C> adj_args(const lbm_desc* d, const Geom& g) {
 }
 
 template <class L, class T, bool FPOST>
-int adjoint_launch(hipStream_t s, int bb, const Geom& g, const AdjArgs<typename Storage<T>::C>& a) {
+int adjoint_launch(hipStream_t s, int bb, bool in_swapped, const Geom& g,
+                   const AdjArgs<typename Storage<T>::C>& a) {
   switch (bb) {
     case LBM_BB_NONE:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_NONE, FPOST>), dim3(g.nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_NONE, FPOST, false>), dim3(g.nblocks), dim3(256), 0, s, a);
       break;
     case LBM_BB_REFERENCE:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_REF, FPOST>), dim3(g.nblocks), dim3(256), 0, s, a);
+      if (in_swapped)
+        hipLaunchKernelGGL((k_adjoint<L, T, BB_REF, FPOST, true>), dim3(g.nblocks), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_adjoint<L, T, BB_REF, FPOST, false>), dim3(g.nblocks), dim3(256), 0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF, FPOST>), dim3(g.nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF, FPOST, false>), dim3(g.nblocks), dim3(256), 0, s, a);
   }
   return check_launch("k_adjoint launch");
 }
@@ -369,8 +394,8 @@ int adjoint_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f
   a.tau_bar = tau_bar;
   a.force_bar = force_bar;
   a.partial = static_cast<double*>(work);
-  if (f_post) return adjoint_launch<L, T, true>(s, d->bb_mode, g, a);
-  return adjoint_launch<L, T, false>(s, d->bb_mode, g, a);
+  if (f_post) return adjoint_launch<L, T, true>(s, d->bb_mode, a.in_swapped != 0, g, a);
+  return adjoint_launch<L, T, false>(s, d->bb_mode, a.in_swapped != 0, g, a);
 }
 
 template <class L, class T>

@@ -199,3 +199,70 @@ def test_float32_adjoint_close_to_float64():
     for s in ("f32", "f32dev"):
         assert rel(grads[s][0], grads["f64"][0]) < 1e-4
         assert rel(grads[s][1], grads["f64"][1]) < 1e-3
+
+
+@pytest.mark.parametrize("lattice,shape", [("D3Q19", (6, 7, 9)), ("D2Q9", (9, 11))])
+@pytest.mark.parametrize("storage", ["f32", "f64"])
+def test_exchanged_slot_cotangents_match_plain_chain(lattice, shape, storage):
+    """Under the reference wall rule, a chain moments-adjoint -> step -> step
+    run with cotangents in the exchanged-slot form (LBM_FLAG_ADJ_*_SWAPPED)
+    gives the plain chain's dL/df bit for bit once the wall slots are
+    exchanged back, and the same dL/dtau."""
+    import ctypes
+    from maddening_amd import _lib
+    from maddening_amd.lbm import LBMKernel, _ptr, _stream
+    D = len(shape)
+    Q = 19 if D == 3 else 9
+    f0, mask, tau, force, wts = setup(lattice, shape, "reference", seed=3)
+    k = LBMKernel(shape, lattice=lattice, tau=0.8, storage=storage, bb="reference")
+    L = _lib.lib()
+    st = _stream(k.device)
+    m = mask.cuda()
+    tf = tau.float().cuda()
+    f = k.from_aos(f0.to(k.device, k.dtype))
+    g1 = k.empty()
+    _lib.check(L.lbm_pull_phase(st, ctypes.byref(k.desc()), 0, _ptr(f), _ptr(g1), _ptr(m), None,
+                                _ptr(tf), None, None, None), "pull")
+    rho_bar = wts["rho"].float().cuda()
+    fbar_N = k.from_aos(wts["f"].to(k.device, k.dtype))
+    opp = torch.as_tensor(k.lat.opp, device=k.device).long()
+    wall = m.bool()
+
+    def chain(swapped):
+        def desc(bits):
+            d = k.desc(force=[1e-4, -2e-4, 3e-4][:D])
+            if swapped:
+                d.flags |= bits
+            return d
+        IN, OUT = _lib.LBM_FLAG_ADJ_IN_SWAPPED, _lib.LBM_FLAG_ADJ_OUT_SWAPPED
+        work = torch.empty(L.lbm_adjoint_workspace_size(ctypes.byref(desc(0))),
+                           dtype=torch.uint8, device=k.device)
+        _lib.check(L.lbm_adjoint_begin(st, ctypes.byref(desc(0)), _ptr(work)), "begin")
+        a, b, c = (torch.empty(k.state_shape, dtype=k.dtype, device=k.device) for _ in range(3))
+        tb = torch.zeros(shape, device=k.device)
+        _lib.check(L.lbm_moments_adjoint(st, ctypes.byref(desc(OUT)), _ptr(g1), _ptr(m), None,
+                                         _ptr(rho_bar), None, None, _ptr(fbar_N), _ptr(a), None,
+                                         _ptr(work)), "moments adjoint")
+        # step 1 started from f_1 = S B g_1 ... here g_1 stands in for the
+        # previous post-collision state; step 0 from f_0 itself
+        _lib.check(L.lbm_adjoint_step(st, ctypes.byref(desc(IN | OUT)), _ptr(g1), 1, _ptr(a),
+                                      _ptr(b), _ptr(m), None, _ptr(tf), _ptr(tb), None,
+                                      _ptr(work)), "step 1")
+        mid = b.clone()
+        _lib.check(L.lbm_adjoint_step(st, ctypes.byref(desc(IN)), _ptr(f), 0, _ptr(b), _ptr(c),
+                                      _ptr(m), None, _ptr(tf), _ptr(tb), None, _ptr(work)),
+                   "step 0")
+        out = torch.empty(3, dtype=torch.float64, device=k.device)
+        _lib.check(L.lbm_adjoint_end(st, ctypes.byref(desc(0)), _ptr(work), _ptr(out)), "end")
+        torch.cuda.synchronize()
+        return mid, c, tb, out
+
+    mid_p, out_p, tb_p, F_p = chain(False)
+    mid_s, out_s, tb_s, F_s = chain(True)
+    # the exchanged form differs from the plain one exactly at wall cells
+    mid_back = torch.where(wall.unsqueeze(0), mid_s[opp], mid_s)
+    assert torch.equal(mid_back, mid_p)
+    assert not torch.equal(mid_s, mid_p)
+    assert torch.equal(out_s, out_p)
+    assert torch.equal(tb_s, tb_p)
+    assert torch.equal(F_s, F_p)
