@@ -86,27 +86,71 @@ def time_kernel(kern, f0, steps, warmup):
     return ev, wall, launches, out
 
 
-def cpu_baseline(target_seconds=15.0):
-    """Float32 NumPy restatement of the reference step (the oracle) on a
-    64^3 Taylor-Green sample; single-threaded NumPy."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(target_seconds=15.0, n=256, numpy_seconds=5.0):
+    """The reference step timed on the host cores, beside the GPU run.
+
+    Main figure: the C restatement of lbm.py:899-986 (oracle/lbm_ref.c, AoS
+    (*G, Q) float32 like the reference's state, OpenMP over cells) on the
+    same 256^3 Taylor-Green workload, for about `target_seconds` of steps.
+    Threads: the cores this process may run on, capped by OMP_NUM_THREADS
+    when that is set. Also reported: the float32 NumPy restatement, which
+    keeps the reference's array-op structure (roll streaming, broadcast Guo
+    source), on one core at 64^3."""
     sys.path.insert(0, ROOT)
+    from oracle import lbm_c as occ
     from oracle import lbm_numpy as orc
     from maddening_amd import workloads
-    n = 64
-    rho, u = workloads.taylor_green_3d(n, 0.05)
+    affinity = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else affinity
     e, w, _ = orc.lattice(19)
-    f = orc.equilibrium(rho, u, e, w).astype(np.float32)
-    orc.update(f, tau=0.8)  # warm-up
-    steps = 0
+    rho, u = workloads.taylor_green_3d(n, 0.05)
+    f = orc.equilibrium(rho.astype(np.float32), u.astype(np.float32), e, w.astype(np.float32))
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    del rho, u
+    # buffers touched before timing, so page faults stay out of the sample
+    bufs = (f.copy(), np.zeros_like(f), np.zeros_like(f))
+    occ.run(f, 1, tau=0.8, nthreads=threads, buffers=bufs)      # warm-up
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < target_seconds:
-        f = orc.update(f, tau=0.8)["f"]
-        steps += 1
+    occ.run(f, 2, tau=0.8, nthreads=threads, buffers=bufs)      # per-step estimate
+    per_step = (time.perf_counter() - t0) / 2
+    steps = max(2, int(target_seconds / max(per_step, 1e-3)))
+    t0 = time.perf_counter()
+    occ.run(f, steps, tau=0.8, nthreads=threads, buffers=bufs)
     dt = time.perf_counter() - t0
-    return {"value": round(n ** 3 * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": 1,
-            "kind": "port",
-            "sample": f"D3Q19 64^3 Taylor-Green, float32 NumPy restatement of lbm.py:899-986 "
-                      f"(AoS, roll streaming), {steps} steps in {dt:.1f} s"}
+    del bufs
+    res = {"value": round(n ** 3 * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads,
+           "kind": "port",
+           "sample": f"D3Q19 {n}^3 Taylor-Green (u0=0.05, tau=0.8), float32, C restatement of "
+                     f"lbm.py:899-986 (AoS, OpenMP over cells), {steps} steps in {dt:.1f} s",
+           "cpu_model": _cpu_model(), "affinity_cores": affinity, "omp_num_threads": omp}
+    del f
+    m = 64
+    rho, u = workloads.taylor_green_3d(m, 0.05)
+    g = orc.equilibrium(rho, u, e, w).astype(np.float32)
+    orc.update(g, tau=0.8)
+    nsteps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < numpy_seconds:
+        g = orc.update(g, tau=0.8)["f"]
+        nsteps += 1
+    dt = time.perf_counter() - t0
+    res["numpy_1_core"] = {
+        "value": round(m ** 3 * nsteps / dt / 1e6, 3), "unit": "MLUPS", "cores": 1,
+        "sample": f"D3Q19 {m}^3 Taylor-Green, float32 NumPy restatement (roll streaming), "
+                  f"{nsteps} steps in {dt:.1f} s"}
+    return res
 
 
 def precision_leg(n, dev, checkpoints=(100, 1000, 5000)):
