@@ -196,11 +196,20 @@ class LBMNode(SimulationNode):
             mask = self._wall_mask
         force = boundary_inputs.get("body_force", None)
         if force is not None:
-            force = torch.as_tensor(force, device=self._device, dtype=torch.float32)
-            if force.ndim == 1:
-                if force.shape != (self._D,):
+            # the reference casts the force to float32 (lbm.py:924-931); a
+            # uniform force given on the host goes to the kernels as numbers
+            # without a trip through device memory
+            if torch.is_tensor(force):
+                force = force.to(device=self._device, dtype=torch.float32)
+                if force.ndim == 1:
+                    force = force.tolist()
+            else:
+                arr = np.asarray(force, dtype=np.float32)
+                force = arr.tolist() if arr.ndim == 1 else torch.as_tensor(arr, device=self._device)
+            if isinstance(force, list):
+                if len(force) != self._D:
                     raise ValueError(f"uniform body_force must have shape ({self._D},)")
-                force = [float(v) for v in force.tolist()]
+                force = [float(v) for v in force]
         # Zou-He faces: rho_p = P / cs2 at the configured faces (lbm.py:950-968)
         faces = None
         p_in = boundary_inputs.get("inlet_pressure")
