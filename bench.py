@@ -372,6 +372,41 @@ def batch_leg(dev, nsteps=200, seed=0):
             "tuned": {"cells_per_thread": tune["cells_per_thread"], "grid": tune["grid"]}}
 
 
+def pipe_leg(dev, shape=(256, 128, 128), nsteps=200, reps=3):
+    """LBMPipeNode (lbm_pipe.py:751-879) on the GPU: single phase with the
+    D3Q7 tracer, and Shan-Chen multiphase, float32, propeller + gravity.
+    Algorithmic bytes per cell and step: single phase 152 (D3Q19) + 56
+    (D3Q7) + 1 (flags) = 209; multiphase 2 passes = 76 + 4 + 1 (psi pass)
+    + 76 + 4 + 76 + 1 (push pass) = 238."""
+    import torch
+    from maddening_amd.pipe import LBMPipeNode
+    nx, ny, nz = shape
+    cells = nx * ny * nz
+    out = {"grid": list(shape), "steps": nsteps}
+    for name, kw, bpc in (("single_phase", dict(fill_fraction=0.75, gravity=-1e-4), 209),
+                          ("multiphase", dict(fill_fraction=0.55, gravity=-1e-3, G=-4.5), 238)):
+        node = LBMPipeNode("fluid", 0.01, nx=nx, ny=ny, nz=nz, propeller_x=nx // 6,
+                           propeller_strength=0.001, device=dev, **kw)
+        st = node.initial_state()
+        node.run(st, {}, 5)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            node.run(st, {}, nsteps)
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1) / 1e3)
+        t = statistics.median(times)
+        out[name] = {"MLUPS": round(cells * nsteps / t / 1e6, 1),
+                     "GB/s": round(bpc * cells * nsteps / t / 1e9, 1),
+                     "ms_per_step": round(t / nsteps * 1e3, 4), "bytes_per_cell": bpc}
+        del node, st
+        torch.cuda.empty_cache()
+    return out
+
+
 def read_traffic(storage):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
@@ -401,10 +436,11 @@ def main():
                          "the default: same bytes as f32, 10x smaller rounding) or f32")
     ap.add_argument("--variants", default="f32,f16dev,bf16dev",
                     help="extra storage types to time (comma list, '' for none)")
-    ap.add_argument("--legs", default="precision,adjoint,batch",
+    ap.add_argument("--legs", default="precision,adjoint,batch,pipe",
                     help="extra measurements: precision (storage types vs float64 at "
                          "100/1000/5000 steps), adjoint (forward+adjoint ms/step, gradient "
-                         "vs central differences), batch (64 uneven lattices)")
+                         "vs central differences), batch (64 uneven lattices), pipe "
+                         "(LBMPipeNode single phase and multiphase)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -494,6 +530,8 @@ def main():
         result["adjoint"] = adjoint_leg(dev)
     if "batch" in legs:
         result["batch"] = batch_leg(dev)
+    if "pipe" in legs:
+        result["pipe"] = pipe_leg(dev)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -286,6 +286,39 @@ int lbm_moments_adjoint(void* stream, const lbm_desc* d, const void* f,
 int lbm_adjoint_end(void* stream, const lbm_desc* d, const void* work,
                     double* force_bar_out);
 
+/* ---- pipe node ----------------------------------------------------------- */
+/*
+ * LBMPipeNode.update (src/maddening/nodes/lbm_pipe.py:751-879): D3Q19 fluid
+ * in a pipe driven by an actuator-disc propeller and gravity. With
+ * multiphase == 0 it carries a D3Q7 passive tracer (BGK, tau_tracer,
+ * collided with the step's output velocity). With multiphase != 0 it adds a
+ * Shan-Chen pseudopotential force with exact-difference forcing, the 0.25
+ * velocity clamp and the 0.01 density floor; the tracer there is derived
+ * from density and tracer_f passes through.
+ *
+ * Unlike the main path, velocities are plain sum(f e)/rho, without the F/2
+ * correction. State: SoA f[19][nx][ny][nz] and g[7][nx][ny][nz], plain
+ * values in f32 or f64 storage. cell_flags[(*G)]: bit 0 wall, bit 1
+ * propeller disc. rho (*G), u (*G, 3) and tracer (*G) are written in the
+ * storage's element type, and each is nullable. work is
+ * lbm_pipe_workspace_size bytes.
+ */
+typedef struct lbm_pipe_desc {
+  int64_t dims[3];         /* nx, ny, nz */
+  int32_t storage;         /* LBM_F32 or LBM_F64 */
+  int32_t multiphase;      /* 0: single phase + tracer; 1: Shan-Chen */
+  double tau, tau_tracer;
+  double propeller_strength; /* x-force on propeller-disc cells */
+  double gravity;          /* z-force on fluid cells */
+  double G, rho_0, rho_wall, rho_liquid, rho_gas;
+  int32_t flags, reserved;
+} lbm_pipe_desc;
+
+size_t lbm_pipe_workspace_size(const lbm_pipe_desc* d);
+int lbm_pipe_step(void* stream, const lbm_pipe_desc* d, const void* f_in, void* f_out,
+                  const void* g_in, void* g_out, const uint8_t* cell_flags, void* work,
+                  void* rho, void* u, void* tracer, int nsteps);
+
 #ifdef __cplusplus
 }
 #endif

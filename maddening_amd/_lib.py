@@ -50,6 +50,26 @@ class LbmDesc(ctypes.Structure):
     ]
 
 
+class LbmPipeDesc(ctypes.Structure):
+    """Mirror of lbm_pipe_desc."""
+    _fields_ = [
+        ("dims", ctypes.c_int64 * 3),
+        ("storage", ctypes.c_int32),
+        ("multiphase", ctypes.c_int32),
+        ("tau", ctypes.c_double),
+        ("tau_tracer", ctypes.c_double),
+        ("propeller_strength", ctypes.c_double),
+        ("gravity", ctypes.c_double),
+        ("G", ctypes.c_double),
+        ("rho_0", ctypes.c_double),
+        ("rho_wall", ctypes.c_double),
+        ("rho_liquid", ctypes.c_double),
+        ("rho_gas", ctypes.c_double),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class LbmBatchItem(ctypes.Structure):
     _fields_ = [
         ("q", ctypes.c_int32),
@@ -100,6 +120,9 @@ _PROTOS = {
                                            _vp, _vp, _vp, _vp, _vp, _vp]),
     "lbm_adjoint_end": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),
     "lbm_conservation": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp, _vp]),
+    "lbm_pipe_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(LbmPipeDesc)]),
+    "lbm_pipe_step": (ctypes.c_int, [_vp, ctypes.POINTER(LbmPipeDesc), _vp, _vp, _vp, _vp, _vp,
+                                     _vp, _vp, _vp, _vp, ctypes.c_int]),
 }
 
 _LIB = None
