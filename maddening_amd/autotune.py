@@ -9,6 +9,7 @@ fastest is stored in a JSON cache keyed by device name and problem shape
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 from typing import Optional
@@ -42,16 +43,20 @@ def _store(key: str, value: dict) -> None:
 
 
 def _time(fn, reps: int) -> float:
+    """Best of ``reps`` timed calls (after one untimed call), in ms: the
+    minimum is the estimate least disturbed by clock and queue noise."""
     fn()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
+    best = float("inf")
     for _ in range(reps):
+        e0.record()
         fn()
-    e1.record()
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / reps
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    return best
 
 
 def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool = True) -> dict:
@@ -88,13 +93,15 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
     return rec
 
 
-def tune_batch(batch, nsteps: int = 4, reps: int = 2, use_cache: bool = True,
+def tune_batch(batch, nsteps: int = 10, reps: int = 3, use_cache: bool = True,
                grids: Optional[list] = None) -> dict:
     """Pick cells_per_thread (tile = 256 * cpt cells) and the persistent grid
     size for a LatticeBatch; re-plans the batch with the winner."""
     dev = torch.cuda.get_device_name(batch.device)
     sig = [(k.lat.name, k.grid_shape, k.bb) for k in batch.kernels]
-    key = f"batch|{dev}|{hash(str(sig)) & 0xffffffff:08x}|{batch.cells}"
+    # a digest that is the same in every process (str hashes are salted)
+    digest = hashlib.sha1(str(sig).encode()).hexdigest()[:12]
+    key = f"batch|{dev}|{digest}|{batch.cells}"
     if use_cache:
         hit = _load().get(key)
         if hit:

@@ -71,7 +71,10 @@ __global__ void __launch_bounds__(256)
       else hi = mid - 1;
     }
     const BatchEntry& e = items[lo];
-    const long long base = (t - e.tile_begin) * (256LL * cpt) + threadIdx.x;
+    // per-lattice cell indices fit int32 (check_desc caps a plane below
+    // 2^31), so the row split is a 32-bit division, not a 64-bit one
+    const int base = (int)(t - e.tile_begin) * (256 * cpt) + (int)threadIdx.x;
+    const int plane = (int)e.plane;
     FwdArgs<float> a = {};
     a.src = e.buf[parity];
     a.dst = e.buf[parity ^ 1];
@@ -85,10 +88,10 @@ __global__ void __launch_bounds__(256)
       a.rho = e.rho; a.u = e.u; a.p = e.p;
     }
     for (int k = 0; k < cpt; ++k) {
-      const long long cell = base + 256LL * k;
-      if (cell >= e.plane) break;
-      const int row = (int)(cell / e.nc);
-      const int ic = (int)(cell - (long long)row * e.nc);
+      const int cell = base + 256 * k;
+      if (cell >= plane) break;
+      const int row = (int)((unsigned)cell / (unsigned)e.nc);
+      const int ic = cell - row * e.nc;
       cell_update<L, float, BB, PULL, COLLIDE, true>(a, row, ic);
     }
   }
