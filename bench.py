@@ -376,15 +376,16 @@ def pipe_leg(dev, shape=(256, 128, 128), nsteps=200, reps=3):
     """LBMPipeNode (lbm_pipe.py:751-879) on the GPU: single phase with the
     D3Q7 tracer, and Shan-Chen multiphase, float32, propeller + gravity.
     Algorithmic bytes per cell and step: single phase 152 (D3Q19) + 56
-    (D3Q7) + 1 (flags) = 209; multiphase 2 passes = 76 + 4 + 1 (psi pass)
-    + 76 + 4 + 76 + 1 (push pass) = 238."""
+    (D3Q7) + 1 (flags) = 209; multiphase, two pull passes: 76 gather + 4 psi
+    + 4 floor scale + 1 flags, then 76 gather + 4 scale + 4 psi + 76 write
+    + 1 flags = 246."""
     import torch
     from maddening_amd.pipe import LBMPipeNode
     nx, ny, nz = shape
     cells = nx * ny * nz
     out = {"grid": list(shape), "steps": nsteps}
     for name, kw, bpc in (("single_phase", dict(fill_fraction=0.75, gravity=-1e-4), 209),
-                          ("multiphase", dict(fill_fraction=0.55, gravity=-1e-3, G=-4.5), 238)):
+                          ("multiphase", dict(fill_fraction=0.55, gravity=-1e-3, G=-4.5), 246)):
         node = LBMPipeNode("fluid", 0.01, nx=nx, ny=ny, nz=nz, propeller_x=nx // 6,
                            propeller_strength=0.001, device=dev, **kw)
         st = node.initial_state()

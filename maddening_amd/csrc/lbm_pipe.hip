@@ -16,12 +16,15 @@
 //                   input and tracer_f_{k-1} = B7 S7 G_{k-2} otherwise.
 //                   k < n collides f_k into F_k; k = n writes f_n and the moments.
 //   tracer launch   tracer_f_n = B7 S7 G_{n-1}, tracer = sum (0 at walls).
-// Multiphase, per step: a psi pass (the density floor of lbm_pipe.py:802-814
-// applied to the state, then psi(rho) with psi_wall at walls) and a push pass
-// (Shan-Chen force from psi at x + e_q, exact-difference forcing, node-swap
-// bounce-back at the destination). A last psi pass writes the outputs
-// (tracer = (rho - rho_gas)/(rho_liquid - rho_gas) clipped to [0, 1]);
-// tracer_f is passed through unchanged (lbm_pipe.py:826-832).
+// Multiphase, also pull with post-collision buffers, two launches per step:
+//   psi pass     gathers f_k = B S F_{k-1} (f_0 itself for k = 0), applies the
+//                density floor of lbm_pipe.py:802-814 as a per-cell scale s
+//                (kept for the next pass) and writes psi(rho) (psi_wall at walls)
+//   collide pass gathers again, scales by s, adds the Shan-Chen force from
+//                psi at x + e_q and collides with exact-difference forcing into F_k
+// and a last launch writes the floored f_n and the outputs (tracer =
+// (rho - rho_gas)/(rho_liquid - rho_gas) clipped to [0, 1]); tracer_f is
+// passed through unchanged (lbm_pipe.py:826-832).
 #include <hip/hip_runtime.h>
 
 #include "../../include/lbm_c_api.h"
@@ -46,13 +49,13 @@ struct PipeArgs {
   T* gdst;
   const uint8_t* flags;
   T* psi;
+  T* scale;
   T* rho;
   T* u;
   T* tracer;
   long long plane;
   int na, nb, nc, bpr;
   int gsrc_aligned;   // tracer_f_{k-1} is the input itself (k = 1)
-  int floor_state;    // psi pass: apply the density floor to the state
   T tau, tau_tracer, strength, gravity;
   T G, rho_0, psi_wall, rho_liquid, rho_gas;
 };
@@ -202,66 +205,65 @@ __device__ __forceinline__ T psi_of(T rho, T rho_0) {
   return rho_0 * (T(1) - exp(-rho / rho_0));
 }
 
-// psi pass over the state fsrc (in place when floor_state); OUT also writes
-// f to fdst (when distinct) and the outputs.
-template <class T, bool OUT>
-__global__ void __launch_bounds__(256) k_pipe_psi(PipeArgs<T> a) {
+// f_k(x) gathered from the post-collision buffer with the node swap at wall
+// cells, as in the single-phase kernel
+template <class T>
+__device__ __forceinline__ void gather19(const T* src, long long P, const Nbr& n, bool wall,
+                                         T (&f)[19]) {
   using L = Lattice<19>;
-  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= a.plane) return;
-  const long long P = a.plane;
-  const bool wall = a.flags[cell] & 1;
-  T* fs = const_cast<T*>(a.fsrc);
-  T f[19];
-  T rho = 0, ma = 0, mb = 0, mc = 0;
 #pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    f[q] = fs[q * P + cell];
-    rho += f[q];
-    if (L::ea[q]) ma += T(L::ea[q]) * f[q];
-    if (L::eb[q]) mb += T(L::eb[q]) * f[q];
-    if (L::ec[q]) mc += T(L::ec[q]) * f[q];
-  }
-  const T inv = T(1) / (rho > T(1e-10) ? rho : T(1e-10));
-  T rho_out = rho;
-  T rho_next = rho;   // what the next step's moments see
-  const bool fix = a.floor_state && rho < T(0.01);
-  if (fix) {
-    const T ratio = (rho > T(0.01) ? rho : T(0.01)) / (rho > T(1e-20) ? rho : T(1e-20));
-    rho_next = 0;
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      f[q] = f[q] * ratio;
-      rho_next += f[q];
-      fs[q * P + cell] = f[q];
-    }
-    rho_out = T(0.01);
-  }
-  if (!OUT) {
-    a.psi[cell] = wall ? a.psi_wall : psi_of<T>(rho_next, a.rho_0);
-    return;
-  }
-  if (a.fdst != fs) {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) a.fdst[q * P + cell] = f[q];
-  }
-  if (a.rho) a.rho[cell] = rho_out;
-  if (a.u) {
-    // velocity of the unscaled state, 0 at walls (lbm_pipe.py:799, 817)
-    a.u[3 * cell + 0] = wall ? T(0) : ma * inv;
-    a.u[3 * cell + 1] = wall ? T(0) : mb * inv;
-    a.u[3 * cell + 2] = wall ? T(0) : mc * inv;
-  }
-  if (a.tracer) {
-    T t = (rho_out - a.rho_gas) / (a.rho_liquid - a.rho_gas);
-    t = t < T(0) ? T(0) : (t > T(1) ? T(1) : t);
-    a.tracer[cell] = wall ? T(0) : t;
-  }
+  for (int q = 0; q < 19; ++q)
+    f[q] = wall ? src[L::opp[q] * P + n.template at<L>(q, +1)]
+                : src[q * P + n.template at<L>(q, -1)];
 }
 
-// collision with Shan-Chen + exact-difference forcing, pushed to x + e_q
-// with the node swap at wall destinations (lbm_pipe.py:770-797, 321-375)
+// Density floor of lbm_pipe.py:802-814 as a per-cell scale: s = 0.01/rho
+// where the streamed density is below 0.01, else 1. The state the next step
+// sees is s * (B S F).
 template <class T>
+__device__ __forceinline__ T floor_scale(T rho) {
+  return rho < T(0.01) ? (rho > T(0.01) ? rho : T(0.01)) / (rho > T(1e-20) ? rho : T(1e-20))
+                       : T(1);
+}
+
+// psi pass of step k: psi(x) from the density of f_k (FIRST: f_0 read in
+// place; otherwise gathered from F_{k-1} and floored, the scale kept for
+// the collision pass).
+template <class T, bool FIRST>
+__global__ void __launch_bounds__(256) k_pipe_psi(PipeArgs<T> a) {
+  int ia, ib, ic;
+  bool valid;
+  cell_index(a, ia, ib, ic, valid);
+  if (!valid) return;
+  Nbr n;
+  n.init(ia, ib, ic, a.na, a.nb, a.nc);
+  const long long P = a.plane;
+  const bool wall = a.flags[n.cell] & 1;
+  T f[19];
+  if (FIRST) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = a.fsrc[q * P + n.cell];
+  } else {
+    gather19<T>(a.fsrc, P, n, wall, f);
+  }
+  T rho = 0;
+#pragma unroll
+  for (int q = 0; q < 19; ++q) rho += f[q];
+  if (!FIRST) {
+    const T s = floor_scale<T>(rho);
+    a.scale[n.cell] = s;
+    if (s != T(1)) {
+      rho = 0;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) rho += f[q] * s;
+    }
+  }
+  a.psi[n.cell] = wall ? a.psi_wall : psi_of<T>(rho, a.rho_0);
+}
+
+// collision of step k with Shan-Chen + exact-difference forcing
+// (lbm_pipe.py:770-787, 321-375), written in place (pull scheme)
+template <class T, bool FIRST>
 __global__ void __launch_bounds__(256) k_pipe_mp(PipeArgs<T> a) {
   using L = Lattice<19>;
   int ia, ib, ic;
@@ -274,10 +276,18 @@ __global__ void __launch_bounds__(256) k_pipe_mp(PipeArgs<T> a) {
   const uint8_t fl = a.flags[n.cell];
   const bool wall = fl & 1;
   T f[19];
+  if (FIRST) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = a.fsrc[q * P + n.cell];
+  } else {
+    gather19<T>(a.fsrc, P, n, wall, f);
+    const T s = a.scale[n.cell];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = s != T(1) ? f[q] * s : f[q];
+  }
   T rho = 0, ma = 0, mb = 0, mc = 0;
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    f[q] = a.fsrc[q * P + n.cell];
     rho += f[q];
     if (L::ea[q]) ma += T(L::ea[q]) * f[q];
     if (L::eb[q]) mb += T(L::eb[q]) * f[q];
@@ -315,10 +325,49 @@ __global__ void __launch_bounds__(256) k_pipe_mp(PipeArgs<T> a) {
   for (int q = 0; q < 19; ++q) {
     const T fb = feq<T>(q, rho, ua, ub, uc, usq);
     const T fsh = feq<T>(q, rho, sa, sb, sc, ssq);
-    const T post = f[q] - (f[q] - fb) / a.tau + (fsh - fb);
-    const int y = n.template at<L>(q, +1);
-    const bool wy = a.flags[y] & 1;
-    a.fdst[(wy ? L::opp[q] : q) * P + y] = post;
+    a.fdst[q * P + n.cell] = f[q] - (f[q] - fb) / a.tau + (fsh - fb);
+  }
+}
+
+// last launch: f_n = floor(B S F_{n-1}) and the outputs; velocity from the
+// unscaled state, 0 at walls; tracer from the floored density
+// (lbm_pipe.py:799-832)
+template <class T>
+__global__ void __launch_bounds__(256) k_pipe_mp_out(PipeArgs<T> a) {
+  using L = Lattice<19>;
+  int ia, ib, ic;
+  bool valid;
+  cell_index(a, ia, ib, ic, valid);
+  if (!valid) return;
+  Nbr n;
+  n.init(ia, ib, ic, a.na, a.nb, a.nc);
+  const long long P = a.plane;
+  const bool wall = a.flags[n.cell] & 1;
+  T f[19];
+  gather19<T>(a.fsrc, P, n, wall, f);
+  T rho = 0, ma = 0, mb = 0, mc = 0;
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    rho += f[q];
+    if (L::ea[q]) ma += T(L::ea[q]) * f[q];
+    if (L::eb[q]) mb += T(L::eb[q]) * f[q];
+    if (L::ec[q]) mc += T(L::ec[q]) * f[q];
+  }
+  const T inv = T(1) / (rho > T(1e-10) ? rho : T(1e-10));
+  const T s = floor_scale<T>(rho);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) a.fdst[q * P + n.cell] = s != T(1) ? f[q] * s : f[q];
+  const T rho_out = s != T(1) ? T(0.01) : rho;
+  if (a.rho) a.rho[n.cell] = rho_out;
+  if (a.u) {
+    a.u[3 * (long long)n.cell + 0] = wall ? T(0) : ma * inv;
+    a.u[3 * (long long)n.cell + 1] = wall ? T(0) : mb * inv;
+    a.u[3 * (long long)n.cell + 2] = wall ? T(0) : mc * inv;
+  }
+  if (a.tracer) {
+    T t = (rho_out - a.rho_gas) / (a.rho_liquid - a.rho_gas);
+    t = t < T(0) ? T(0) : (t > T(1) ? T(1) : t);
+    a.tracer[n.cell] = wall ? T(0) : t;
   }
 }
 
@@ -355,8 +404,8 @@ inline int pipe_geom(const lbm_pipe_desc* d, PipeGeom* g) {
 inline size_t pipe_work_bytes(const lbm_pipe_desc* d, const PipeGeom& g) {
   const size_t s = d->storage == LBM_F64 ? 8 : 4;
   // single phase: F ping-pong (2 x 19 planes) + G ping-pong (2 x 7 planes);
-  // multiphase: state ping-pong (2 x 19 planes) + psi (1 plane)
-  const size_t planes = d->multiphase ? 39 : 52;
+  // multiphase: F ping-pong (2 x 19 planes) + psi + floor scale
+  const size_t planes = d->multiphase ? 40 : 52;
   return planes * (size_t)g.plane * s;
 }
 
@@ -412,24 +461,24 @@ int pipe_run(hipStream_t s, const lbm_pipe_desc* d, const PipeGeom& g, const T* 
   }
   T* S[2] = {work, work + 19 * P};
   a.psi = work + 38 * P;
-  const dim3 grid1((unsigned)((P + 255) / 256));
-  const T* cur = f_in;
+  a.scale = work + 39 * P;
   for (int k = 0; k < nsteps; ++k) {
-    a.fsrc = cur;
-    a.floor_state = k > 0;    // the input state is taken as given
-    hipLaunchKernelGGL((k_pipe_psi<T, false>), grid1, block, 0, s, a);
-    T* next = k == nsteps - 1 ? f_out : S[k & 1];
-    a.fdst = next;
-    hipLaunchKernelGGL((k_pipe_mp<T>), grid, block, 0, s, a);
-    cur = next;
+    a.fsrc = k == 0 ? f_in : S[(k - 1) & 1];   // f_0, or F_{k-1}
+    a.fdst = S[k & 1];                          // F_k
+    if (k == 0) {
+      hipLaunchKernelGGL((k_pipe_psi<T, true>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((k_pipe_mp<T, true>), grid, block, 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_pipe_psi<T, false>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((k_pipe_mp<T, false>), grid, block, 0, s, a);
+    }
   }
-  a.fsrc = f_out;
+  a.fsrc = S[(nsteps - 1) & 1];
   a.fdst = f_out;
-  a.floor_state = 1;
   a.rho = rho;
   a.u = u;
   a.tracer = tracer;
-  hipLaunchKernelGGL((k_pipe_psi<T, true>), grid1, block, 0, s, a);
+  hipLaunchKernelGGL((k_pipe_mp_out<T>), grid, block, 0, s, a);
   if (g_out != g_in) {
     if (hipMemcpyAsync(g_out, g_in, 7 * (size_t)P * sizeof(T), hipMemcpyDeviceToDevice, s) !=
         hipSuccess)
