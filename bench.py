@@ -189,7 +189,7 @@ def precision_leg(n, dev, checkpoints=(100, 1000, 5000)):
     return out
 
 
-def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=4, reps=3):
+def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=16, reps=3):
     """Sphere-packed channel (porosity 0.7, radii 6-12, seed 0), tau field 0.7,
     body force (0, 0, 1e-5), loss = mean u_z over the outlet plane. Times the
     float32 forward and the sqrt(N)-checkpointed adjoint; checks the float64
@@ -372,6 +372,42 @@ def batch_leg(dev, nsteps=200, seed=0):
             "tuned": {"cells_per_thread": tune["cells_per_thread"], "grid": tune["grid"]}}
 
 
+def d2q9_leg(dev, n=8192, nsteps=200, reps=3):
+    """D2Q9 periodic Taylor-Green at n x n (67 M cells at 8192^2): the 2-D
+    path's throughput for each storage type. Algorithmic bytes per cell
+    2 * 9 * s_f (72 B for 32-bit storage, 36 B for 16-bit)."""
+    import torch
+    from maddening_amd import workloads
+    from maddening_amd.lbm import LBMKernel
+    rho, u = workloads.taylor_green_2d(n, n, 0.05)
+    rho = torch.from_numpy(rho).float().to(dev)
+    u = torch.from_numpy(u).float().to(dev)
+    out = {"grid": [n, n], "steps": nsteps}
+    for storage in ("f32dev", "f16dev"):
+        k = LBMKernel((n, n), lattice="D2Q9", tau=0.8, storage=storage, device=dev)
+        f0 = k.equilibrium(rho, u)
+        work = k.workspace(nsteps)
+        res = k.empty()
+        k.step(f0, 5, out=res, work=work, moments=False)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            k.step(f0, nsteps, out=res, work=work, moments=False)
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1) / 1e3)
+        t = statistics.median(times)
+        bpc = 2 * 9 * (2 if storage == "f16dev" else 4)
+        out[storage] = {"MLUPS": round(n * n * nsteps / t / 1e6, 1),
+                        "GB/s": round(bpc * n * n * (nsteps + 1) / t / 1e9, 1),
+                        "bytes_per_cell": bpc}
+        del k, f0, work, res
+        torch.cuda.empty_cache()
+    return out
+
+
 def pipe_leg(dev, shape=(256, 128, 128), nsteps=200, reps=3):
     """LBMPipeNode (lbm_pipe.py:751-879) on the GPU: single phase with the
     D3Q7 tracer, and Shan-Chen multiphase, float32, propeller + gravity.
@@ -437,11 +473,11 @@ def main():
                          "the default: same bytes as f32, 10x smaller rounding) or f32")
     ap.add_argument("--variants", default="f32,f16dev,bf16dev",
                     help="extra storage types to time (comma list, '' for none)")
-    ap.add_argument("--legs", default="precision,adjoint,batch,pipe",
+    ap.add_argument("--legs", default="precision,adjoint,batch,pipe,d2q9",
                     help="extra measurements: precision (storage types vs float64 at "
                          "100/1000/5000 steps), adjoint (forward+adjoint ms/step, gradient "
                          "vs central differences), batch (64 uneven lattices), pipe "
-                         "(LBMPipeNode single phase and multiphase)")
+                         "(LBMPipeNode single phase and multiphase), d2q9 (8192^2 D2Q9)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -533,6 +569,8 @@ def main():
         result["batch"] = batch_leg(dev)
     if "pipe" in legs:
         result["pipe"] = pipe_leg(dev)
+    if "d2q9" in legs:
+        result["d2q9"] = d2q9_leg(dev)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(result), flush=True)

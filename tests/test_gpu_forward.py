@@ -282,3 +282,29 @@ def test_zou_he_faces_f32_periodic_row_path_off():
     g, o = run_both(shape, "D3Q19", 0.7, 10, storage="f32", scheme="pull", row_vec=1,
                     faces=(("z_min", 1.01), None))
     assert rel(g["density"], o["density"]) < 1e-5
+
+
+# degenerate and ragged extents: axes of length 1 and 2 (periodic neighbours
+# wrap onto the cell itself or onto the same partner from both sides), rows
+# shorter than a wave and not a multiple of 64, prime extents
+DEGENERATE = [("D3Q19", (1, 1, 1)), ("D3Q19", (1, 5, 3)), ("D3Q19", (2, 1, 7)),
+              ("D3Q19", (3, 2, 1)), ("D3Q19", (7, 11, 13)), ("D3Q19", (4, 3, 257)),
+              ("D2Q9", (1, 1)), ("D2Q9", (1, 9)), ("D2Q9", (13, 1)), ("D2Q9", (2, 2)),
+              ("D2Q9", (5, 131))]
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("lattice,shape", DEGENERATE)
+def test_degenerate_extents_f64(lattice, shape, scheme):
+    rng = np.random.default_rng(sum(shape))
+    D = len(shape)
+    rho = 1 + 0.02 * rng.standard_normal(shape)
+    u = 0.03 * rng.standard_normal(shape + (D,))
+    force = 1e-4 * rng.standard_normal(shape + (D,))
+    mask = (rng.random(shape) < 0.2).astype(np.uint8)
+    for bb, m in (("none", None), ("reference", mask)):
+        gpu, ref = run_both(shape, lattice, 0.77, 6, rho0=rho, u0=u, storage="f64",
+                            scheme=scheme, bb=bb, mask=m, force=force)
+        for key in ("f", "density", "velocity"):
+            assert rel(gpu[key], ref[key]) < 1e-12 if key == "f" else \
+                rel(gpu[key], ref[key]) < 1e-6, (bb, key, rel(gpu[key], ref[key]))
