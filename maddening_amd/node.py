@@ -225,11 +225,35 @@ class LBMNode(SimulationNode):
 
     def run(self, state: dict, boundary_inputs: dict, nsteps: int) -> dict:
         """``nsteps`` updates with the same boundary inputs in one kernel
-        sequence; equal to calling ``update`` nsteps times."""
+        sequence; equal to calling ``update`` nsteps times.
+
+        Differentiable: when ``state["f"]`` or a tensor ``body_force``
+        requires grad (and grad mode is on), the step runs as an autograd
+        rollout whose backward is the HIP adjoint (sqrt(N) checkpointing). That
+        is the counterpart of jax.grad through the reference's update."""
         mask, force, faces = self._inputs(state, boundary_inputs)
         k = self._kernel
-        f_new, rho, u, p = k.step(self._soa(state["f"]), nsteps, mask=mask.to(torch.uint8),
-                                  force=force, faces=faces)
+        raw_force = boundary_inputs.get("body_force")
+        wants_grad = torch.is_grad_enabled() and (
+            (torch.is_tensor(state["f"]) and state["f"].requires_grad)
+            or (torch.is_tensor(raw_force) and raw_force.requires_grad))
+        if wants_grad:
+            if faces is not None:
+                raise NotImplementedError("gradients through Zou-He pressure faces are not "
+                                          "available")
+            from .adjoint import rollout
+            gforce = None
+            if raw_force is not None:
+                gforce = raw_force if torch.is_tensor(raw_force) else torch.as_tensor(
+                    np.asarray(raw_force, np.float32), device=self._device)
+                # float32 values as on the non-differentiable path (and in the
+                # reference); the cast passes gradients through unchanged
+                gforce = gforce.to(self._device).float()
+            f_new, rho, u, p = rollout(k, self._soa(state["f"]), nsteps, force=gforce,
+                                       mask=mask.to(torch.uint8))
+        else:
+            f_new, rho, u, p = k.step(self._soa(state["f"]), nsteps, mask=mask.to(torch.uint8),
+                                      force=force, faces=faces)
         result = {
             "f": k.aos_view(f_new),
             "density": rho.to(self._dtype),
