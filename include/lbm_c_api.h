@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LBM_API_VERSION 3
+#define LBM_API_VERSION 4
 
 typedef enum lbm_status {
   LBM_OK = 0,
@@ -101,6 +101,12 @@ typedef struct lbm_desc {
  * fbar_in, OUT to fbar_out. */
 #define LBM_FLAG_ADJ_IN_SWAPPED 2
 #define LBM_FLAG_ADJ_OUT_SWAPPED 4
+/* Adjoint with Zou-He faces: fbar_out is passed back through the transpose
+ * of the faces that produced the step's starting state (that is, it is the
+ * cotangent with respect to the state before those faces) and dL/drho of
+ * each face is added to work. Set it on every lbm_adjoint_step of a chain
+ * except the one that returns dL/df_0, and on lbm_moments_adjoint. */
+#define LBM_FLAG_ADJ_OUT_FACES 8
 
 /* ---- queries ----------------------------------------------------------- */
 
@@ -285,6 +291,12 @@ int lbm_moments_adjoint(void* stream, const lbm_desc* d, const void* f,
 
 int lbm_adjoint_end(void* stream, const lbm_desc* d, const void* work,
                     double* force_bar_out);
+
+/* dL/drho_in, dL/drho_out of the Zou-He faces (prescribed densities, i.e.
+ * pressure / cs2) accumulated in work by calls with LBM_FLAG_ADJ_OUT_FACES:
+ * rho_bar_out[0] inlet, [1] outlet (0 where a face is not configured). */
+int lbm_adjoint_face_end(void* stream, const lbm_desc* d, const void* work,
+                         double* rho_bar_out);
 
 /* ---- pipe node ----------------------------------------------------------- */
 /*

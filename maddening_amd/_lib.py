@@ -20,6 +20,7 @@ LBM_SCHEME_AUTO, LBM_SCHEME_PULL, LBM_SCHEME_PUSH = 0, 1, 2
 LBM_FLAG_XCD_REMAP = 1
 LBM_FLAG_ADJ_IN_SWAPPED = 2
 LBM_FLAG_ADJ_OUT_SWAPPED = 4
+LBM_FLAG_ADJ_OUT_FACES = 8
 
 STORAGE_NAMES = {"f32": LBM_F32, "f64": LBM_F64, "f16dev": LBM_F16_DEV,
                  "bf16dev": LBM_BF16_DEV, "f32dev": LBM_F32_DEV}
@@ -120,6 +121,7 @@ _PROTOS = {
                                            _vp, _vp, _vp, _vp, _vp, _vp]),
     "lbm_adjoint_end": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),
     "lbm_conservation": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp, _vp]),
+    "lbm_adjoint_face_end": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),
     "lbm_pipe_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(LbmPipeDesc)]),
     "lbm_pipe_step": (ctypes.c_int, [_vp, ctypes.POINTER(LbmPipeDesc), _vp, _vp, _vp, _vp, _vp,
                                      _vp, _vp, _vp, _vp, ctypes.c_int]),

@@ -50,7 +50,8 @@ class _Rollout(torch.autograd.Function):
     at step s is therefore checkpointed by keeping f_0 (s = 0) or g_{s-1}."""
 
     @staticmethod
-    def forward(ctx, f0, force, tau_field, kern: LBMKernel, nsteps: int, mask, seg: int):
+    def forward(ctx, f0, force, tau_field, rho_in, rho_out, kern: LBMKernel, nsteps: int, mask,
+                seg: int, face_names):
         ctx.set_materialize_grads(False)
         L = _lib.lib()
         dev = kern.device
@@ -58,7 +59,8 @@ class _Rollout(torch.autograd.Function):
         # a uniform force goes to the kernels as host numbers: read it once
         fk = force.tolist() if (force is not None and force.dim() == 1) else force
         ff, fu = kern._field(fk)
-        d = kern.desc(force=fu)
+        faces = _faces(face_names, rho_in, rho_out)
+        d = kern.desc(force=fu, faces=faces)
         m = kern._mask(mask)
         if kern.bb != "none" and m is None:
             raise ValueError(f"bb='{kern.bb}' needs a wall mask")
@@ -92,6 +94,9 @@ class _Rollout(torch.autograd.Function):
         ctx.force_dtype = None if force is None else force.dtype
         ctx.force_list = fk if isinstance(fk, list) else None
         ctx.n_ckpt = len(ckpts)
+        ctx.faces = faces
+        ctx.rho_dtypes = (None if rho_in is None else rho_in.dtype,
+                          None if rho_out is None else rho_out.dtype)
         ctx.save_for_backward(fN, force, tau_field, *ckpts, *tail)
         return fN, rho, u, p
 
@@ -107,7 +112,7 @@ class _Rollout(torch.autograd.Function):
             ff, fu = None, ctx.force_list
         else:
             ff, fu = kern._field(force)
-        d = kern.desc(force=fu)
+        d = kern.desc(force=fu, faces=ctx.faces)
         m = kern._mask(ctx.mask)
         tf = None if tau_field is None else tau_field.to(dev, torch.float32).contiguous()
         need_f0, need_force, need_tau = ctx.needs_input_grad[0], ctx.needs_input_grad[1], \
@@ -127,13 +132,20 @@ class _Rollout(torch.autograd.Function):
         # with the slots of wall cells exchanged, so each transposed step is a
         # plain gather (no neighbour mask reads); the last step writes dL/df_0
         # in the ordinary form.
-        def with_flags(bits):
+        # With Zou-He faces every cotangent but dL/df_0 is passed back through
+        # the faces' transpose (their states came out of the faces), which
+        # also accumulates dL/drho of each face.
+        faces_on = ctx.faces is not None
+        OUTF = _lib.LBM_FLAG_ADJ_OUT_FACES if faces_on else 0
+
+        def with_flags(bits, extra=0):
             dd = _lib.LbmDesc.from_buffer_copy(d)
             if kern.bb == "reference":
                 dd.flags |= bits
+            dd.flags |= extra
             return dd
-        d_mom = with_flags(_lib.LBM_FLAG_ADJ_OUT_SWAPPED)
-        d_mid = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED | _lib.LBM_FLAG_ADJ_OUT_SWAPPED)
+        d_mom = with_flags(_lib.LBM_FLAG_ADJ_OUT_SWAPPED, OUTF)
+        d_mid = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED | _lib.LBM_FLAG_ADJ_OUT_SWAPPED, OUTF)
         d_first = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED)
         fbar = torch.empty(kern.state_shape, dtype=kern.dtype, device=dev)
         gin = None if g_f is None else g_f.to(dev, kern.dtype).contiguous()
@@ -174,17 +186,40 @@ class _Rollout(torch.autograd.Function):
                            "lbm_adjoint_end")
                 grad_force = out[: kern.D].to(ctx.force_dtype)
         grad_tau = tau_bar.to(tau_field.dtype) if tau_bar is not None else None
-        return (fbar if need_f0 else None), grad_force, grad_tau, None, None, None, None
+        grad_rin = grad_rout = None
+        if faces_on and (ctx.needs_input_grad[3] or ctx.needs_input_grad[4]):
+            rb = torch.empty(2, dtype=torch.float64, device=dev)
+            _lib.check(L.lbm_adjoint_face_end(st, ctypes.byref(d), _ptr(work), _ptr(rb)),
+                       "lbm_adjoint_face_end")
+            if ctx.needs_input_grad[3]:
+                grad_rin = rb[0].to(ctx.rho_dtypes[0])
+            if ctx.needs_input_grad[4]:
+                grad_rout = rb[1].to(ctx.rho_dtypes[1])
+        return ((fbar if need_f0 else None), grad_force, grad_tau, grad_rin, grad_rout,
+                None, None, None, None, None)
+
+
+def _faces(names, rho_in, rho_out):
+    """((inlet, rho_in), (outlet, rho_out)) for LBMKernel.desc, or None."""
+    if names is None:
+        return None
+    vals = [None if r is None else float(r) for r in (rho_in, rho_out)]
+    out = tuple(None if n is None else (n, v) for n, v in zip(names, vals))
+    return None if out == (None, None) else out
 
 
 def rollout(kern: LBMKernel, f0: torch.Tensor, nsteps: int, *,
             force: Optional[torch.Tensor] = None, tau_field: Optional[torch.Tensor] = None,
-            mask: Optional[torch.Tensor] = None, checkpoint: Union[str, int] = "sqrt"):
+            mask: Optional[torch.Tensor] = None, checkpoint: Union[str, int] = "sqrt",
+            faces=None):
     """Differentiable ``nsteps``-step rollout; returns (f_N, rho, u, p).
 
     ``force`` is None, a (D,) tensor (uniform) or a (*G, D) tensor;
     ``tau_field`` is None or a (*G) tensor; either may require grad, as may
-    ``f0``. Needs f32 or f64 storage.
+    ``f0``. ``faces``: Zou-He pressure faces ((inlet_face, rho_in),
+    (outlet_face, rho_out)), either entry None, with each rho (= pressure /
+    cs2) a float or a 0-dim tensor that may require grad. Needs f32, f32dev
+    or f64 storage.
     """
     if kern.storage not in ("f32", "f32dev", "f64"):
         raise ValueError("differentiable rollouts need f32, f32dev or f64 storage")
@@ -193,4 +228,15 @@ def rollout(kern: LBMKernel, f0: torch.Tensor, nsteps: int, *,
     if force is not None and not torch.is_tensor(force):
         force = torch.as_tensor(force, dtype=torch.float64, device=kern.device)
     seg = segment_length(nsteps, checkpoint)
-    return _Rollout.apply(f0, force, tau_field, kern, int(nsteps), mask, seg)
+    names, rho_in, rho_out = None, None, None
+    if faces is not None:
+        names = tuple(None if fc is None else fc[0] for fc in faces)
+
+        def as_t(fc):
+            if fc is None:
+                return None
+            v = fc[1]
+            return v if torch.is_tensor(v) else torch.tensor(float(v), dtype=torch.float64)
+        rho_in, rho_out = as_t(faces[0]), as_t(faces[1])
+    return _Rollout.apply(f0, force, tau_field, rho_in, rho_out, kern, int(nsteps), mask, seg,
+                          names)

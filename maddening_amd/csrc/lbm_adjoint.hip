@@ -40,32 +40,52 @@ struct AdjArgs {
   int bpr, nblocks, remap;
   int out_swapped;      // write fbar_out with the slots of wall cells exchanged
   int in_swapped;       // fbar_in of the moments adjoint is in that form
+  int out_faces;        // apply the transpose of the Zou-He faces to fbar_out
+  // Zou-He faces, as in the forward arguments (lbm_cell.hpp FwdArgs)
+  int zh_n;
+  int zh_axis[2];
+  int zh_side[2];
+  int zh_slot[2];       // 0 inlet, 1 outlet: where dL/drho of face i goes
+  C zh_rho[2];
   C inv_tau;
   C Fa, Fb, Fc;
   C lid_a, lid_b, lid_c;
 };
 
-// Sum three values over the workgroup and add them to partial[4*lb + k].
-template <int NT>
-__device__ __forceinline__ void block_accumulate(double v0, double v1, double v2, double* partial,
-                                                 int lb) {
-  __shared__ double sh[3][NT];
-  sh[0][threadIdx.x] = v0;
-  sh[1][threadIdx.x] = v1;
-  sh[2][threadIdx.x] = v2;
+// Per-workgroup partial sums: kPart doubles per workgroup, slots 0-2 dL/dF
+// of a uniform force, slots 3-4 dL/drho of the inlet and outlet faces.
+constexpr int kPart = 8;
+
+// Sum NV values over the workgroup and add them to partial[kPart*lb + k].
+template <int NT, int NV>
+__device__ __forceinline__ void block_accumulate(const double (&v)[NV], double* partial, int lb) {
+  __shared__ double sh[NV][NT];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = v[k];
   __syncthreads();
   for (int w = NT / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
-      sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partial[4 * (long long)lb + 0] += sh[0][0];
-    partial[4 * (long long)lb + 1] += sh[1][0];
-    partial[4 * (long long)lb + 2] += sh[2][0];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partial[kPart * (long long)lb + k] += sh[k][0];
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void accumulate_partials(double dFa, double dFb, double dFc,
+                                                    double r0, double r1, bool faces,
+                                                    double* partial, int lb) {
+  if (faces) {
+    const double v[5] = {dFa, dFb, dFc, r0, r1};
+    block_accumulate<NT, 5>(v, partial, lb);
+  } else {
+    const double v[3] = {dFa, dFb, dFc};
+    block_accumulate<NT, 3>(v, partial, lb);
   }
 }
 
@@ -105,7 +125,9 @@ __device__ __forceinline__ void read_force(const AdjArgs<C>& a, int cell, C& Fa,
 // cell exchanged, so the transposed node swap folds into a plain gather and
 // no neighbour mask is read. a.out_swapped writes gout in the same form,
 // with this cell's own mask.
-template <class L, class T, int BB, bool FPOST, bool GSW>
+// FACES: Zou-He faces are configured (a.zh_n > 0); the face-free kernels
+// carry none of that code.
+template <class L, class T, int BB, bool FPOST, bool GSW, bool FACES>
 __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C> a) {
   using C = typename Storage<T>::C;
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
@@ -113,6 +135,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
   const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
   const bool valid = ic < a.nc;
   C dFa = 0, dFb = 0, dFc = 0;
+  double rb0 = 0, rb1 = 0;
   if (valid) {
     const int ia = row / a.nb;
     const int ib = row - ia * a.nb;
@@ -146,6 +169,8 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       C d[L::Q];
       gather_state<L, T, BB, FPOST>(static_cast<const T*>(a.f), a.mask, P, n, self, a.lid_a,
                                     a.lid_b, a.lid_c, d);
+      // the state the step started from includes the previous step's faces
+      if (FPOST && FACES && !self) apply_faces<L, C>(a, ia, ib, ic, d);
       C Fa, Fb, Fc;
       read_force<L, C>(a, n.cell, Fa, Fb, Fc);
       C drho, rho, ua, ub, uc;
@@ -208,11 +233,26 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       const C keep = C(1) - om;
       // exchanged form: slot p carries the cotangent of opp(p), whose e is -e_p
       const bool xs = BB == BB_REF && a.out_swapped && self;
+      if (FACES && a.out_faces && !self) {
+        // f_n itself came out of the previous step's faces: pass the
+        // cotangent back through their transpose
+        C o[L::Q];
 #pragma unroll
-      for (int p = 0; p < L::Q; ++p) {
-        const C eB = C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
-        const C g = xs ? gb[L::opp[p]] : gb[p];
-        a.gout[p * P + n.cell] = keep * g + A + (xs ? -eB : eB);
+        for (int p = 0; p < L::Q; ++p)
+          o[p] = keep * gb[p] + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
+        C rbf[2] = {C(0), C(0)};
+        apply_faces_transpose<L, C>(a, ia, ib, ic, o, rbf);
+        rb0 = double(rbf[0]);
+        rb1 = double(rbf[1]);
+#pragma unroll
+        for (int p = 0; p < L::Q; ++p) a.gout[p * P + n.cell] = o[p];
+      } else {
+#pragma unroll
+        for (int p = 0; p < L::Q; ++p) {
+          const C eB = C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
+          const C g = xs ? gb[L::opp[p]] : gb[p];
+          a.gout[p * P + n.cell] = keep * g + A + (xs ? -eB : eB);
+        }
       }
       if (a.tau_bar) a.tau_bar[n.cell] += float(-om * om * tb);
       // dL/dF: the source directly (u fixed) plus through u = (j + F/2)/rho
@@ -222,7 +262,9 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       if (a.force_bar) add_force_bar<L, C>(a, n.cell, dFa, dFb, dFc);
     }
   }
-  if (a.partial) block_accumulate<256>(double(dFa), double(dFb), double(dFc), a.partial, lb);
+  if (a.partial)
+    accumulate_partials<256>(double(dFa), double(dFb), double(dFc), rb0, rb1,
+                             FACES && a.out_faces, a.partial, lb);
 }
 
 // Cotangent of the output moments (lbm.py:970-976) pulled back onto f:
@@ -235,6 +277,7 @@ __global__ void __launch_bounds__(256) k_moments_adjoint(AdjArgs<typename Storag
   const int lb = blockIdx.x;
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   C dFa = 0, dFb = 0, dFc = 0;
+  double rb0 = 0, rb1 = 0;
   if (cell < a.plane) {
     const long long P = a.plane;
     const T* f = static_cast<const T*>(a.f);
@@ -266,17 +309,38 @@ __global__ void __launch_bounds__(256) k_moments_adjoint(AdjArgs<typename Storag
     // only the incoming cotangent: read from opp(p) when exactly one of the
     // two buffers is in the exchanged form
     const bool cross = wall && (a.in_swapped != a.out_swapped);
+    if (a.out_faces && !wall) {
+      // f was written by the faces of the last step: transpose them too
+      C o[L::Q];
 #pragma unroll
-    for (int p = 0; p < L::Q; ++p) {
-      const C base = a.gin ? a.gin[(cross ? L::opp[p] : p) * P + cell] : C(0);
-      a.gout[p * P + cell] = base + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
+      for (int p = 0; p < L::Q; ++p) {
+        const C base = a.gin ? a.gin[p * P + cell] : C(0);
+        o[p] = base + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
+      }
+      const int ic = (int)(cell % a.nc);
+      const int row = (int)(cell / a.nc);
+      const int ib = row % a.nb, ia = row / a.nb;
+      C rbf[2] = {C(0), C(0)};
+      apply_faces_transpose<L, C>(a, ia, ib, ic, o, rbf);
+      rb0 = double(rbf[0]);
+      rb1 = double(rbf[1]);
+#pragma unroll
+      for (int p = 0; p < L::Q; ++p) a.gout[p * P + cell] = o[p];
+    } else {
+#pragma unroll
+      for (int p = 0; p < L::Q; ++p) {
+        const C base = a.gin ? a.gin[(cross ? L::opp[p] : p) * P + cell] : C(0);
+        a.gout[p * P + cell] = base + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
+      }
     }
     dFa = C(0.5) * Ba;
     dFb = C(0.5) * Bb;
     dFc = C(0.5) * Bc;
     if (a.force_bar) add_force_bar<L, C>(a, (int)cell, dFa, dFb, dFc);
   }
-  if (a.partial) block_accumulate<256>(double(dFa), double(dFb), double(dFc), a.partial, lb);
+  if (a.partial)
+    accumulate_partials<256>(double(dFa), double(dFb), double(dFc), rb0, rb1,
+                             a.zh_n > 0 && a.out_faces, a.partial, lb);
 }
 
 __global__ void __launch_bounds__(256) k_fill_zero(double* p, long long n) {
@@ -285,14 +349,18 @@ __global__ void __launch_bounds__(256) k_fill_zero(double* p, long long n) {
     p[i] = 0.0;
 }
 
-template <int D>
+// Fold the per-workgroup partials: MODE 0 gives dL/dF (slots 0-2; D2Q9's
+// axes (b, c) are the reference's (0, 1)), MODE 1 dL/drho of the two faces
+// (slots 3-4).
+template <int D, int MODE>
 __global__ void __launch_bounds__(256) k_partial_sum(const double* partial, long long nparts,
                                                      double* out) {
   __shared__ double sh[3][256];
   double s[3] = {0, 0, 0};
+  const int off = MODE == 0 ? 0 : 3;
+  const int nv = MODE == 0 ? 3 : 2;
   for (long long i = threadIdx.x; i < nparts; i += blockDim.x)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) s[k] += partial[4 * i + k];
+    for (int k = 0; k < nv; ++k) s[k] += partial[kPart * i + off + k];
 #pragma unroll
   for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = s[k];
   __syncthreads();
@@ -303,11 +371,14 @@ __global__ void __launch_bounds__(256) k_partial_sum(const double* partial, long
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    if (D == 3) {
+    if (MODE == 1) {
+      out[0] = sh[0][0];
+      out[1] = sh[1][0];
+    } else if (D == 3) {
       out[0] = sh[0][0];
       out[1] = sh[1][0];
       out[2] = sh[2][0];
-    } else {  // D2Q9 axes (b, c) are the reference's (0, 1)
+    } else {
       out[0] = sh[1][0];
       out[1] = sh[2][0];
       out[2] = 0.0;
@@ -322,8 +393,6 @@ inline int adj_geom(const lbm_desc* d, Geom* g) {
   if (rc) return rc;
   if (d->storage != LBM_F32 && d->storage != LBM_F64 && d->storage != LBM_F32_DEV)
     return fail(LBM_ERR_UNSUPPORTED, "the adjoint needs f32, f32dev or f64 storage");
-  if (d->inlet_face > 0 || d->outlet_face > 0)
-    return fail(LBM_ERR_UNSUPPORTED, "the adjoint does not cover Zou-He pressure faces");
   if (d->bb_mode == LBM_BB_HALFWAY && (d->flags & (LBM_FLAG_ADJ_IN_SWAPPED | LBM_FLAG_ADJ_OUT_SWAPPED)))
     return fail(LBM_ERR_UNSUPPORTED,
                 "exchanged-slot cotangents are defined for the reference wall rule only");
@@ -349,6 +418,19 @@ AdjArgs<C> adj_args(const lbm_desc* d, const Geom& g) {
   // the exchanged form only means something where walls swap slots
   a.in_swapped = (d->bb_mode == LBM_BB_REFERENCE && (d->flags & LBM_FLAG_ADJ_IN_SWAPPED)) ? 1 : 0;
   a.out_swapped = (d->bb_mode == LBM_BB_REFERENCE && (d->flags & LBM_FLAG_ADJ_OUT_SWAPPED)) ? 1 : 0;
+  a.out_faces = (d->flags & LBM_FLAG_ADJ_OUT_FACES) ? 1 : 0;
+  // faces in application order (inlet, outlet), axes as in the forward
+  const int faces[2] = {d->inlet_face, d->outlet_face};
+  const double rhos[2] = {d->inlet_rho, d->outlet_rho};
+  a.zh_n = 0;
+  for (int i = 0; i < 2; ++i) {
+    if (faces[i] <= 0) continue;
+    a.zh_axis[a.zh_n] = (faces[i] - 1) / 2 + (d->q == 19 ? 0 : 1);
+    a.zh_side[a.zh_n] = (faces[i] - 1) % 2;
+    a.zh_slot[a.zh_n] = i;
+    a.zh_rho[a.zh_n] = C(rhos[i]);
+    ++a.zh_n;
+  }
   a.inv_tau = C(1.0 / d->tau);
   if (d->q == 19) {
     a.Fa = C(d->force[0]); a.Fb = C(d->force[1]); a.Fc = C(d->force[2]);
@@ -360,21 +442,25 @@ AdjArgs<C> adj_args(const lbm_desc* d, const Geom& g) {
   return a;
 }
 
-template <class L, class T, bool FPOST>
+template <class L, class T, bool FPOST, bool FACES>
 int adjoint_launch(hipStream_t s, int bb, bool in_swapped, const Geom& g,
                    const AdjArgs<typename Storage<T>::C>& a) {
   switch (bb) {
     case LBM_BB_NONE:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_NONE, FPOST, false>), dim3(g.nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_NONE, FPOST, false, FACES>), dim3(g.nblocks),
+                         dim3(256), 0, s, a);
       break;
     case LBM_BB_REFERENCE:
       if (in_swapped)
-        hipLaunchKernelGGL((k_adjoint<L, T, BB_REF, FPOST, true>), dim3(g.nblocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_adjoint<L, T, BB_REF, FPOST, true, FACES>), dim3(g.nblocks),
+                           dim3(256), 0, s, a);
       else
-        hipLaunchKernelGGL((k_adjoint<L, T, BB_REF, FPOST, false>), dim3(g.nblocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_adjoint<L, T, BB_REF, FPOST, false, FACES>), dim3(g.nblocks),
+                           dim3(256), 0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF, FPOST, false>), dim3(g.nblocks), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF, FPOST, false, FACES>), dim3(g.nblocks),
+                         dim3(256), 0, s, a);
   }
   return check_launch("k_adjoint launch");
 }
@@ -394,8 +480,13 @@ int adjoint_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f
   a.tau_bar = tau_bar;
   a.force_bar = force_bar;
   a.partial = static_cast<double*>(work);
-  if (f_post) return adjoint_launch<L, T, true>(s, d->bb_mode, a.in_swapped != 0, g, a);
-  return adjoint_launch<L, T, false>(s, d->bb_mode, a.in_swapped != 0, g, a);
+  const bool sw = a.in_swapped != 0;
+  if (a.zh_n > 0) {
+    if (f_post) return adjoint_launch<L, T, true, true>(s, d->bb_mode, sw, g, a);
+    return adjoint_launch<L, T, false, true>(s, d->bb_mode, sw, g, a);
+  }
+  if (f_post) return adjoint_launch<L, T, true, false>(s, d->bb_mode, sw, g, a);
+  return adjoint_launch<L, T, false, false>(s, d->bb_mode, sw, g, a);
 }
 
 template <class L, class T>
@@ -429,7 +520,7 @@ extern "C" {
 size_t lbm_adjoint_workspace_size(const lbm_desc* d) {
   Geom g;
   if (adj_geom(d, &g)) return 0;
-  return (size_t)partial_slots(g) * 4 * sizeof(double);
+  return (size_t)partial_slots(g) * kPart * sizeof(double);
 }
 
 int lbm_adjoint_begin(void* stream, const lbm_desc* d, void* work) {
@@ -439,7 +530,7 @@ int lbm_adjoint_begin(void* stream, const lbm_desc* d, void* work) {
   int rc = adj_geom(d, &g);
   if (rc) return rc;
   if (!work) return fail(LBM_ERR_INVALID, "work must be non-NULL");
-  const long long n = partial_slots(g) * 4;
+  const long long n = partial_slots(g) * kPart;
   hipLaunchKernelGGL(k_fill_zero, dim3(256), dim3(256), 0, (hipStream_t)stream,
                      static_cast<double*>(work), n);
   return check_launch("k_fill_zero launch");
@@ -518,11 +609,23 @@ int lbm_adjoint_end(void* stream, const lbm_desc* d, const void* work, double* f
   if (rc) return rc;
   if (!work || !force_bar_out) return fail(LBM_ERR_INVALID, "work and force_bar_out must be non-NULL");
   if (d->q == 19)
-    hipLaunchKernelGGL(k_partial_sum<3>, dim3(1), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL((k_partial_sum<3, 0>), dim3(1), dim3(256), 0, (hipStream_t)stream,
                        static_cast<const double*>(work), partial_slots(g), force_bar_out);
   else
-    hipLaunchKernelGGL(k_partial_sum<2>, dim3(1), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL((k_partial_sum<2, 0>), dim3(1), dim3(256), 0, (hipStream_t)stream,
                        static_cast<const double*>(work), partial_slots(g), force_bar_out);
+  return check_launch("k_partial_sum launch");
+}
+
+int lbm_adjoint_face_end(void* stream, const lbm_desc* d, const void* work, double* rho_bar_out) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = adj_geom(d, &g);
+  if (rc) return rc;
+  if (!work || !rho_bar_out) return fail(LBM_ERR_INVALID, "work and rho_bar_out must be non-NULL");
+  hipLaunchKernelGGL((k_partial_sum<3, 1>), dim3(1), dim3(256), 0, (hipStream_t)stream,
+                     static_cast<const double*>(work), partial_slots(g), rho_bar_out);
   return check_launch("k_partial_sum launch");
 }
 

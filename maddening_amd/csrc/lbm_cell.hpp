@@ -63,14 +63,71 @@ __device__ __forceinline__ void zou_he(C (&d)[L::Q], int axis, int side, C rho_p
 }
 
 // Apply the configured faces to a cell at (ia, ib, ic) that is not a wall.
-template <class L, class C>
-__device__ __forceinline__ void apply_faces(const FwdArgs<C>& a, int ia, int ib, int ic,
-                                            C (&d)[L::Q]) {
+// A is any argument block with the zh_* face fields and the extents.
+template <class L, class C, class A>
+__device__ __forceinline__ void apply_faces(const A& a, int ia, int ib, int ic, C (&d)[L::Q]) {
   for (int i = 0; i < a.zh_n; ++i) {
     const int ax = a.zh_axis[i];
     const int coord = ax == 0 ? ia : (ax == 1 ? ib : ic);
     const int n = ax == 0 ? a.na : (ax == 1 ? a.nb : a.nc);
     if (coord == (a.zh_side[i] == 0 ? 0 : n - 1)) zou_he<L, C>(d, ax, a.zh_side[i], a.zh_rho[i]);
+  }
+}
+
+// Transpose of zou_he on a cotangent y (w.r.t. its output) at one cell; the
+// face map is affine in the populations (its Jacobian depends only on rho_p):
+//   unknown q:   new_q = d_opp(q) + 6 w_q rho_p e_qn u_n,
+//                u_n = s (1 - S / r), s = +1 (first layer) or -1, r = max(rho_p, 1e-10),
+//                S = sum of f over known and tangential slots,
+// so d new_q / d d_p = [p == opp q] - 6 s w_q e_qn rho_p / r for every known or
+// tangential p, unknown slots are overwritten (no dependence on their old
+// value), and d new_q / d rho_p = 6 s w_q e_qn. Returns y mapped to the
+// cotangent w.r.t. the face map's input, adds dL/drho_p to rho_bar.
+template <class L, class C>
+__device__ __forceinline__ void zou_he_transpose(C (&y)[L::Q], int axis, int side, C rho_p,
+                                                 C& rho_bar) {
+  const C r = rho_p > C(1e-10) ? rho_p : C(1e-10);
+  const C sg = side == 0 ? C(1) : C(-1);
+  C c = 0, rb = 0;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int e = axis == 0 ? L::ea[q] : (axis == 1 ? L::eb[q] : L::ec[q]);
+    const bool unknown = side == 0 ? e > 0 : e < 0;
+    if (unknown) {
+      const C k6 = C(6) * weight<L, C>(q) * C(e);
+      c = fmadd(y[q], -sg * k6 * rho_p / r, c);
+      rb = fmadd(y[q], sg * k6, rb);
+    }
+  }
+  C x[L::Q];
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int e = axis == 0 ? L::ea[q] : (axis == 1 ? L::eb[q] : L::ec[q]);
+    const bool unknown = side == 0 ? e > 0 : e < 0;
+    x[q] = unknown ? C(0) : y[q] + c;
+  }
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    const int e = axis == 0 ? L::ea[q] : (axis == 1 ? L::eb[q] : L::ec[q]);
+    const bool unknown = side == 0 ? e > 0 : e < 0;
+    if (unknown) x[L::opp[q]] += y[q];
+  }
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) y[q] = x[q];
+  rho_bar += rb;
+}
+
+// Transpose of apply_faces (faces in reverse order); rho_bar[a.zh_slot[i]]
+// += dL/drho_p of face i (slot 0 inlet, 1 outlet).
+template <class L, class C, class A>
+__device__ __forceinline__ void apply_faces_transpose(const A& a, int ia, int ib, int ic,
+                                                      C (&y)[L::Q], C (&rho_bar)[2]) {
+  for (int i = a.zh_n - 1; i >= 0; --i) {
+    const int ax = a.zh_axis[i];
+    const int coord = ax == 0 ? ia : (ax == 1 ? ib : ic);
+    const int n = ax == 0 ? a.na : (ax == 1 ? a.nb : a.nc);
+    if (coord == (a.zh_side[i] == 0 ? 0 : n - 1))
+      zou_he_transpose<L, C>(y, ax, a.zh_side[i], a.zh_rho[i], rho_bar[a.zh_slot[i]]);
   }
 }
 

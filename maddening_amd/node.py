@@ -227,21 +227,28 @@ class LBMNode(SimulationNode):
         """``nsteps`` updates with the same boundary inputs in one kernel
         sequence; equal to calling ``update`` nsteps times.
 
-        Differentiable: when ``state["f"]`` or a tensor ``body_force``
-        requires grad (and grad mode is on), the step runs as an autograd
-        rollout whose backward is the HIP adjoint (sqrt(N) checkpointing). That
-        is the counterpart of jax.grad through the reference's update."""
+        Differentiable: when ``state["f"]``, a tensor ``body_force`` or a
+        tensor ``inlet_pressure``/``outlet_pressure`` requires grad (and grad
+        mode is on), the step runs as an autograd rollout whose backward is
+        the HIP adjoint (sqrt(N) checkpointing). That is the counterpart of
+        jax.grad through the reference's update."""
         mask, force, faces = self._inputs(state, boundary_inputs)
         k = self._kernel
         raw_force = boundary_inputs.get("body_force")
+        raw_p = (boundary_inputs.get("inlet_pressure"), boundary_inputs.get("outlet_pressure"))
         wants_grad = torch.is_grad_enabled() and (
             (torch.is_tensor(state["f"]) and state["f"].requires_grad)
-            or (torch.is_tensor(raw_force) and raw_force.requires_grad))
+            or (torch.is_tensor(raw_force) and raw_force.requires_grad)
+            or any(torch.is_tensor(v) and v.requires_grad for v in raw_p))
         if wants_grad:
-            if faces is not None:
-                raise NotImplementedError("gradients through Zou-He pressure faces are not "
-                                          "available")
             from .adjoint import rollout
+            gfaces = None
+            if faces is not None:
+                # rho_p = P / cs2, kept differentiable when P is a tensor
+                def rho_of(v):
+                    return v / self._cs2 if torch.is_tensor(v) else float(v) / self._cs2
+                gfaces = (None if raw_p[0] is None else (self._inlet_face, rho_of(raw_p[0])),
+                          None if raw_p[1] is None else (self._outlet_face, rho_of(raw_p[1])))
             gforce = None
             if raw_force is not None:
                 gforce = raw_force if torch.is_tensor(raw_force) else torch.as_tensor(
@@ -250,7 +257,7 @@ class LBMNode(SimulationNode):
                 # reference); the cast passes gradients through unchanged
                 gforce = gforce.to(self._device).float()
             f_new, rho, u, p = rollout(k, self._soa(state["f"]), nsteps, force=gforce,
-                                       mask=mask.to(torch.uint8))
+                                       mask=mask.to(torch.uint8), faces=gfaces)
         else:
             f_new, rho, u, p = k.step(self._soa(state["f"]), nsteps, mask=mask.to(torch.uint8),
                                       force=force, faces=faces)

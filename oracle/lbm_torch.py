@@ -59,8 +59,44 @@ def stream(f, e_int):
     return torch.stack(outs, -1)
 
 
-def update(f, *, tau, force, mask=None, bb="reference", lid_velocity=None):
-    """One step on an AoS (*G, Q) float64 tensor; returns (f, rho, u)."""
+def zou_he_pressure_face(f, rho_p, e, w, opp, face, wall):
+    """Non-equilibrium bounce-back on one face (lbm.py:425-510), with the
+    reference's normal velocity u_n = +-(1 - (sum known + sum tang)/rho_p);
+    rho_p may be a float or a 0-dim tensor (then it is differentiable)."""
+    axis, side = _np_oracle.FACES[face]
+    D = e.shape[1]
+    known, unknown, tang = _np_oracle.classify_directions(e.cpu().numpy().astype(int), axis, side)
+    idx = [slice(None)] * D
+    idx[axis] = 0 if side == "min" else -1
+    idx = tuple(idx)
+    ff = f[idx]
+    wf = wall[idx]
+    s = ff[..., known].sum(-1) + ff[..., tang].sum(-1)
+    rho_p = torch.as_tensor(rho_p, dtype=f.dtype, device=f.device)
+    r = torch.clamp(rho_p, min=1e-10)
+    un = 1.0 - s / r if side == "min" else -1.0 + s / r
+    vel = torch.zeros(ff.shape[:-1] + (D,), dtype=f.dtype, device=f.device)
+    vel = torch.cat([vel[..., :axis], un[..., None], vel[..., axis + 1:]], -1)
+    feq = equilibrium(rho_p.expand(ff.shape[:-1]), vel, e, w)
+    cols = []
+    for q in range(f.shape[-1]):
+        if q in unknown:
+            o = int(opp[q])
+            cand = ff[..., o] + feq[..., q] - feq[..., o]
+            cols.append(torch.where(wf, ff[..., q], cand))
+        else:
+            cols.append(ff[..., q])
+    new = torch.stack(cols, -1)
+    out = f.clone()
+    out[idx] = new
+    return out
+
+
+def update(f, *, tau, force, mask=None, bb="reference", lid_velocity=None, inlet=None,
+           outlet=None):
+    """One step on an AoS (*G, Q) float64 tensor; returns (f, rho, u).
+    inlet/outlet: optional (face, rho_p) Zou-He faces (lbm.py:950-968), with
+    rho_p = pressure / cs2 a float or a 0-dim tensor."""
     Q = f.shape[-1]
     e, w, opp, e_int = lattice(Q, f.device)
     G = f.shape[:-1]
@@ -90,6 +126,9 @@ def update(f, *, tau, force, mask=None, bb="reference", lid_velocity=None):
         out = torch.where(wall[..., None], w.expand(out.shape), out)
     else:
         out = torch.where(wall[..., None], f_str[..., opp], f_str)
+    for fc in (inlet, outlet):
+        if fc is not None:
+            out = zou_he_pressure_face(out, fc[1], e, w, opp, fc[0], wall)
     rho_n, u_n = macroscopic(out, e, force)
     u_n = torch.where(wall[..., None], 0.0, u_n)
     return out, rho_n, u_n

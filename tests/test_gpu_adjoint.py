@@ -266,3 +266,76 @@ def test_exchanged_slot_cotangents_match_plain_chain(lattice, shape, storage):
     assert torch.equal(out_s, out_p)
     assert torch.equal(tb_s, tb_p)
     assert torch.equal(F_s, F_p)
+
+
+FACE_CASES = [("D2Q9", (10, 7), (("x_min", 1.03), ("x_max", 0.99)), "reference"),
+              ("D3Q19", (6, 7, 8), (("z_min", 1.02), ("y_max", 0.98)), "reference"),
+              ("D3Q19", (5, 6, 7), (None, ("x_max", 0.97)), "none")]
+
+
+@pytest.mark.parametrize("lattice,shape,faces,bb", FACE_CASES)
+@pytest.mark.parametrize("nsteps,checkpoint", [(1, "all"), (6, "sqrt"), (6, "all")])
+def test_adjoint_through_zou_he_faces_matches_torch_autograd(lattice, shape, faces, bb, nsteps,
+                                                             checkpoint):
+    """Rollouts with Zou-He faces (pressure = rho cs2 on two faces, a shared
+    edge in 3-D, or one face only) against autograd of the torch
+    restatement: dL/df0, dL/dF and dL/drho of each face, f64 storage."""
+    from maddening_amd.adjoint import rollout
+    from maddening_amd.lbm import LBMKernel
+    D = len(shape)
+    f0, mask, _, force, wts = setup(lattice, shape, bb, seed=21)
+    Fu = force.reshape(-1, D).mean(0)          # a uniform force
+    # oracle
+    fo = f0.clone().requires_grad_(True)
+    Fo = Fu.clone().requires_grad_(True)
+    ro = [None if fc is None else torch.tensor(fc[1], dtype=torch.float64, requires_grad=True)
+          for fc in faces]
+    f = fo
+    for _ in range(nsteps):
+        f, r, u = ot.update(f, tau=0.8, force=Fo, mask=mask, bb=bb,
+                            inlet=None if faces[0] is None else (faces[0][0], ro[0]),
+                            outlet=None if faces[1] is None else (faces[1][0], ro[1]))
+    Lo = (f * wts["f"]).sum() + (r * wts["rho"]).sum() + (u * wts["u"]).sum()
+    Lo.backward()
+    # HIP
+    k = LBMKernel(shape, lattice=lattice, tau=0.8, storage="f64", bb=bb)
+    fg = k.from_aos(f0.cuda()).requires_grad_(True)
+    Fg = Fu.clone().cuda().requires_grad_(True)
+    rg = [None if fc is None else torch.tensor(fc[1], dtype=torch.float64, requires_grad=True)
+          for fc in faces]
+    gf = tuple(None if fc is None else (fc[0], rgi) for fc, rgi in zip(faces, rg))
+    f1, r1, u1, _ = rollout(k, fg, nsteps, force=Fg, mask=None if mask is None else mask.cuda(),
+                            checkpoint=checkpoint, faces=gf)
+    Lg = (soa_to_aos(f1, D).cpu() * wts["f"]).sum() + (r1.double().cpu() * wts["rho"]).sum() \
+        + (u1.double().cpu() * wts["u"]).sum()
+    Lg.backward()
+    assert rel(Lg.detach(), Lo.detach()) < 1e-6
+    assert rel(soa_to_aos(fg.grad, D), fo.grad) < 1e-10
+    assert rel(Fg.grad.cpu(), Fo.grad) < 1e-9
+    for a, b in zip(rg, ro):
+        if a is not None:
+            assert rel(a.grad, b.grad) < 1e-9, (float(a.grad), float(b.grad))
+
+
+def test_node_pressure_gradient_matches_finite_differences():
+    """d(mean outlet-region velocity)/d(inlet pressure) through the GPU node
+    (reference Poiseuille channel, lbm.py:950-968) equals central differences."""
+    from maddening_amd.node import LBMNode
+    nx, ny = 32, 12
+    wall = np.zeros((nx, ny), dtype=bool)
+    wall[:, 0] = wall[:, -1] = True
+    node = LBMNode("ch", 1e-3, grid_shape=(nx, ny), viscosity=0.1, lattice="D2Q9",
+                   wall_mask=wall, inlet_face="x_min", outlet_face="x_max", storage="f64")
+
+    def loss(p_in, grad):
+        with torch.set_grad_enabled(grad):
+            st = node.run(node.initial_state(), {"inlet_pressure": p_in,
+                                                 "outlet_pressure": 1.0 / 3.0}, 200)
+            return st["f"][nx // 2, :, 1].sum() - st["f"][nx // 2, :, 2].sum()   # x-momentum
+
+    p = torch.tensor(1.0 / 3.0 * 1.01, dtype=torch.float64, requires_grad=True)
+    loss(p, True).backward()
+    h = 1e-6
+    fd = (float(loss(float(p.detach()) + h, False)) - float(loss(float(p.detach()) - h, False))) \
+        / (2 * h)
+    assert abs(float(p.grad) - fd) < 1e-6 * abs(fd), (float(p.grad), fd)

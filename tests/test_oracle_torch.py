@@ -64,3 +64,44 @@ def test_torch_restatement_gradient_vs_central_differences():
         tm[c] -= h
         fd = (loss(tp, F) - loss(tm, F)) / (2 * h)
         assert float(t.grad[c]) == pytest.approx(float(fd), rel=1e-5, abs=1e-9)
+
+
+@pytest.mark.parametrize("shape,faces", [((12, 9), (("x_min", 1.03), ("x_max", 0.99))),
+                                         ((6, 7, 8), (("z_min", 1.02), ("y_max", 0.98)))])
+def test_torch_zou_he_faces_match_numpy_and_rho_gradient(shape, faces):
+    """Zou-He faces in the torch restatement: values equal the NumPy oracle's
+    (which takes pressure = rho cs2), and d/d rho_p of a loss equals central
+    differences."""
+    from oracle import lbm_numpy as on
+    rng = np.random.default_rng(11)
+    Q = 19 if len(shape) == 3 else 9
+    e, w, _ = on.lattice(Q)
+    D = len(shape)
+    f0 = on.equilibrium(1 + 0.01 * rng.random(shape), 0.02 * rng.standard_normal(shape + (D,)),
+                        e, w)
+    mask = (rng.random(shape) < 0.15).astype(np.uint8)
+    a = on.update(f0, tau=0.8, mask=mask, inlet=(faces[0][0], faces[0][1] / 3),
+                  outlet=(faces[1][0], faces[1][1] / 3))
+    zero = torch.zeros(D, dtype=torch.float64)
+    b, _, _ = ot.update(torch.from_numpy(f0), tau=0.8, force=zero, mask=torch.from_numpy(mask),
+                        inlet=faces[0], outlet=faces[1])
+    assert np.abs(a["f"] - b.numpy()).max() < 1e-15
+    wts = torch.from_numpy(rng.standard_normal(shape + (Q,)))
+
+    def loss(r_in, r_out):
+        out, rho, u = ot.update(torch.from_numpy(f0), tau=0.8, force=zero,
+                                mask=torch.from_numpy(mask), inlet=(faces[0][0], r_in),
+                                outlet=(faces[1][0], r_out))
+        return (out * wts).sum() + (u * wts[..., :D]).sum()
+
+    r = [torch.tensor(faces[0][1], dtype=torch.float64, requires_grad=True),
+         torch.tensor(faces[1][1], dtype=torch.float64, requires_grad=True)]
+    loss(*r).backward()
+    h = 1e-6
+    for i in range(2):
+        plus = [float(x.detach()) for x in r]
+        minus = [float(x.detach()) for x in r]
+        plus[i] += h
+        minus[i] -= h
+        fd = (float(loss(*plus)) - float(loss(*minus))) / (2 * h)
+        assert abs(float(r[i].grad) - fd) < 1e-6 * max(1.0, abs(fd)), (i, float(r[i].grad), fd)
