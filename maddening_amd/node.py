@@ -215,8 +215,10 @@ class LBMNode(SimulationNode):
         p_in = boundary_inputs.get("inlet_pressure")
         p_out = boundary_inputs.get("outlet_pressure")
         if p_in is not None or p_out is not None:
-            faces = (None if p_in is None else (self._inlet_face, float(p_in) / self._cs2),
-                     None if p_out is None else (self._outlet_face, float(p_out) / self._cs2))
+            def num(v):
+                return float(v.detach()) if torch.is_tensor(v) else float(v)
+            faces = (None if p_in is None else (self._inlet_face, num(p_in) / self._cs2),
+                     None if p_out is None else (self._outlet_face, num(p_out) / self._cs2))
         return mask, force, faces
 
     def update(self, state: dict, boundary_inputs: dict, dt: float) -> dict:
