@@ -90,3 +90,28 @@ def test_desc_layout_matches_c(tmp_path):
     D = _lib.LbmDesc
     assert vals == [ctypes.sizeof(D), D.dims.offset, D.tau.offset, D.force.offset,
                     D.lid_velocity.offset, D.block_x.offset, D.flags.offset, D.scheme.offset]
+
+
+def _c_layout(tmp_path, struct, fields):
+    src = tmp_path / f"{struct}.c"
+    fmt = " ".join(["%zu"] * (len(fields) + 1))
+    args = ", ".join([f"sizeof({struct})"] + [f"offsetof({struct}, {f})" for f in fields])
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lbm_c_api.h"\n'
+                   f'int main(void){{printf("{fmt}\\n", {args});return 0;}}\n')
+    exe = tmp_path / struct
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    return [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+
+
+@pytest.mark.parametrize("struct,cls", [("lbm_desc", "LbmDesc"), ("lbm_pipe_desc", "LbmPipeDesc"),
+                                        ("lbm_batch_item", "LbmBatchItem")])
+def test_every_struct_layout_matches_c(tmp_path, struct, cls):
+    """Size and every field offset of each ABI struct, from gcc, against the
+    ctypes mirror in maddening_amd/_lib.py."""
+    from maddening_amd import _lib
+    C = getattr(_lib, cls)
+    names = [f[0] for f in C._fields_]
+    vals = _c_layout(tmp_path, struct, names)
+    assert vals == [ctypes.sizeof(C)] + [getattr(C, n).offset for n in names]
