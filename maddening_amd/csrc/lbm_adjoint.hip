@@ -161,8 +161,10 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
         } else if (BB == BB_REF) {
           gb[p] = a.gin[(a.mask[x] ? L::opp[p] : p) * P + x];
         } else {
-          const bool hit = (p != 0) && a.mask[x] != 0;
-          gb[p] = hit ? a.gin[L::opp[p] * P + n.cell] : a.gin[p * P + x];
+          // streamed slot loaded first (valid address either way), so only
+          // blocked links wait for the neighbour's mask
+          gb[p] = a.gin[p * P + x];
+          if (p != 0 && a.mask[x] != 0) gb[p] = a.gin[L::opp[p] * P + n.cell];
         }
       }
       // forward quantities of this cell

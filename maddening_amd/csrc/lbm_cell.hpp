@@ -205,7 +205,11 @@ __device__ __forceinline__ void gather_state(const T* src, const uint8_t* mask, 
                           : src + q * P + n.template at<L>(q, -1);
       d[q] = St::template load<L>(ptr, q);
     } else if (BB == BB_HALF) {
+      // the streamed value is loaded whatever the neighbour is (its address
+      // is valid either way), so the common fluid-fluid link does not wait
+      // for the mask; a blocked link then takes the cell's own reflected value
       const int s = n.template at<L>(q, -1);
+      d[q] = St::template load<L>(src + q * P + s, q);
       const uint8_t m = (q == 0) ? uint8_t(0) : mask[s];
       if (m) {
         // 2 w_q (e_q . u_wall) / cs2: momentum a moving wall (mask 2) adds to
@@ -214,8 +218,6 @@ __device__ __forceinline__ void gather_state(const T* src, const uint8_t* mask, 
                                    (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c)
                              : C(0);
         d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) + lid;
-      } else {
-        d[q] = St::template load<L>(src + q * P + s, q);
       }
     } else {
       d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
