@@ -116,7 +116,9 @@ int lbm_version(void);
 /* Message for the last failing call on this thread ("" if none). */
 const char* lbm_last_error(void);
 
-/* Bytes of device workspace lbm_step() needs for `nsteps` steps (0 if none). */
+/* Bytes of device workspace lbm_step() needs for `nsteps` steps (0 if none):
+ * one state for the pull sequence's ping-pong, plus 4 bytes per cell of
+ * bounce-back link words under LBM_BB_HALFWAY. */
 size_t lbm_workspace_size(const lbm_desc* d, int nsteps);
 
 /* Bytes of one population state (Q * cells * storage size). */

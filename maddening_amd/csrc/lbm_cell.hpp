@@ -14,6 +14,10 @@ struct FwdArgs {
   const void* src;
   void* dst;
   const uint8_t* mask;
+  // per-cell link words for halfway bounce-back (k_links), or null: bit q
+  // set when the cell is fluid and x - e_q is a wall, kLinkMoving when one of
+  // those walls moves, kLinkSolid for a wall cell
+  const uint32_t* links;
   const float* force_field;
   const float* tau_field;
   float* rho;
@@ -33,6 +37,9 @@ struct FwdArgs {
   int zh_side[2];  // 0: first layer, 1: last layer
   C zh_rho[2];     // prescribed density P / cs2
 };
+
+constexpr uint32_t kLinkSolid = 1u << 31;
+constexpr uint32_t kLinkMoving = 1u << 30;
 
 // Zou-He pressure face at one fluid cell (lbm.py:425-510), on deviations.
 // Directions pointing out of the domain through the face are unknown; the
@@ -225,6 +232,41 @@ __device__ __forceinline__ void gather_state(const T* src, const uint8_t* mask, 
   }
 }
 
+// Halfway bounce-back gather from a link word (see FwdArgs::links): the same
+// values as gather_state<L, T, BB_HALF, true>, with one aligned 4-byte read
+// in place of the cell's own and its 18 neighbours' mask bytes. A wave with
+// no blocked link skips the wall code entirely; only a link to a moving wall
+// reads that wall's mask byte.
+template <class L, class T>
+__device__ __forceinline__ void gather_links(const T* src, const uint8_t* mask, long long P,
+                                             const Nbr& n, uint32_t lk,
+                                             typename Storage<T>::C lid_a,
+                                             typename Storage<T>::C lid_b,
+                                             typename Storage<T>::C lid_c,
+                                             typename Storage<T>::C (&d)[L::Q]) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  if (lk & kLinkSolid) {
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) d[q] = C(0);
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
+  if (lk) {
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) {
+      if ((lk >> q) & 1u) {
+        C lid = C(0);
+        if ((lk & kLinkMoving) && mask[n.template at<L>(q, -1)] == 2)
+          lid = C(6) * weight<L, C>(q) *
+                (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c);
+        d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) + lid;
+      }
+    }
+  }
+}
+
 // PULL: gather (stream + bounce-back) instead of an aligned read.
 // COLLIDE: apply the collision after the read.
 // STORE: write the state back (false for the moments-only pass).
@@ -239,10 +281,16 @@ __device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C
   n.init(ia, ib, ic, a.na, a.nb, a.nc);
   const T* src = static_cast<const T*>(a.src);
   const long long P = a.plane;
-  const uint8_t self = a.mask ? a.mask[n.cell] : uint8_t(0);
-
   C d[L::Q];
-  gather_state<L, T, BB, PULL>(src, a.mask, P, n, self, a.lid_a, a.lid_b, a.lid_c, d);
+  uint8_t self;
+  if (BB == BB_HALF && PULL && a.links) {
+    const uint32_t lk = a.links[n.cell];
+    self = (lk & kLinkSolid) ? uint8_t(1) : uint8_t(0);
+    gather_links<L, T>(src, a.mask, P, n, lk, a.lid_a, a.lid_b, a.lid_c, d);
+  } else {
+    self = a.mask ? a.mask[n.cell] : uint8_t(0);
+    gather_state<L, T, BB, PULL>(src, a.mask, P, n, self, a.lid_a, a.lid_b, a.lid_c, d);
+  }
   if (PULL && a.zh_n && !self) apply_faces<L, C>(a, ia, ib, ic, d);
 
   C Fa, Fb, Fc;

@@ -381,9 +381,42 @@ __global__ void __launch_bounds__(256) k_conserve_final(const double* partial, i
   }
 }
 
+// Link words for halfway bounce-back, once per lbm_step call (the mask is an
+// input of each call): reads the cell's mask byte and its neighbours' (from
+// cache), writes 4 bytes per cell.
+template <class L>
+__global__ void __launch_bounds__(256)
+    k_links(const uint8_t* __restrict__ mask, uint32_t* __restrict__ links, int na, int nb, int nc,
+            int bpr) {
+  const int row = blockIdx.x / bpr;
+  const int ic = (blockIdx.x - row * bpr) * blockDim.x + threadIdx.x;
+  if (ic >= nc) return;
+  const int ia = row / nb;
+  const int ib = row - ia * nb;
+  Nbr n;
+  n.init(ia, ib, ic, na, nb, nc);
+  uint32_t w = 0;
+  if (mask[n.cell]) {
+    w = kLinkSolid;
+  } else {
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) {
+      const uint8_t m = mask[n.template at<L>(q, -1)];
+      if (m) w |= (1u << q) | (m == 2 ? kLinkMoving : 0u);
+    }
+  }
+  links[n.cell] = w;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+
+// Byte offset of the link words in lbm_step's workspace (after one state).
+inline size_t links_offset(const lbm_desc* d, const Geom& g) {
+  const size_t state = (size_t)g.plane * (size_t)d->q * (size_t)storage_size(d->storage);
+  return (state + 255) & ~(size_t)255;
+}
 
 template <class C>
 FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
@@ -537,7 +570,14 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
     switch (d->bb_mode) {
       case LBM_BB_NONE: return step_pull<L, T, BB_NONE>(s, gr, a, f_in, f_out, work, nsteps, rho, u, p);
       case LBM_BB_REFERENCE: return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
-      default: return step_pull<L, T, BB_HALF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+      default: {
+        uint32_t* links = reinterpret_cast<uint32_t*>(static_cast<char*>(work) + links_offset(d, g));
+        hipLaunchKernelGGL((k_links<L>), dim3(g.nblocks), dim3(g.bx), 0, s, mask, links, g.na,
+                           g.nb, g.nc, g.bpr);
+        if (int rc = check_launch("k_links launch")) return rc;
+        a.links = links;
+        return step_pull<L, T, BB_HALF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+      }
     }
   }
   switch (d->bb_mode) {
@@ -700,6 +740,8 @@ size_t lbm_workspace_size(const lbm_desc* d, int nsteps) {
   if (check_desc(d, &g)) return 0;
   const int scheme = resolve_scheme(d, nsteps);
   if (scheme == LBM_SCHEME_PUSH && nsteps <= 1) return 0;
+  if (scheme == LBM_SCHEME_PULL && d->bb_mode == LBM_BB_HALFWAY)
+    return links_offset(d, g) + (size_t)g.plane * sizeof(uint32_t);
   return (size_t)g.plane * (size_t)d->q * (size_t)storage_size(d->storage);
 }
 

@@ -308,3 +308,70 @@ def test_degenerate_extents_f64(lattice, shape, scheme):
         for key in ("f", "density", "velocity"):
             assert rel(gpu[key], ref[key]) < 1e-12 if key == "f" else \
                 rel(gpu[key], ref[key]) < 1e-6, (bb, key, rel(gpu[key], ref[key]))
+
+
+def _halfway_walls(shape, seed):
+    """Random static (1) and moving (2) wall cells."""
+    rng = np.random.default_rng(seed)
+    r = rng.random(shape)
+    return np.where(r < 0.15, 1, np.where(r < 0.25, 2, 0)).astype(np.uint8)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("shape", [(12, 10, 9), (3, 17, 5)])
+def test_d3q19_halfway_moving_walls_f64(scheme, shape):
+    """Halfway bounce-back with static and moving wall cells scattered through
+    the lattice, a per-cell force and a lid velocity on every axis."""
+    rng = np.random.default_rng(11)
+    mask = _halfway_walls(shape, 5)
+    force = 1e-4 * rng.standard_normal(shape + (3,))
+    f0 = orc.equilibrium(1 + 0.01 * rng.standard_normal(shape),
+                         0.02 * rng.standard_normal(shape + (3,)), *orc.lattice(19)[:2])
+    g, o = run_both(shape, "D3Q19", 0.9, 9, storage="f64", scheme=scheme, bb="halfway",
+                    mask=mask, force=force, f0_aos=f0, lid=(0.03, -0.02, 0.05))
+    fluid = mask == 0
+    assert rel(g["f"][fluid], o["f"][fluid]) < 1e-12
+    assert np.all(g["velocity"][~fluid] == 0.0)
+    assert rel(g["velocity"], o["velocity"]) < 1e-6
+
+
+@pytest.mark.parametrize("lattice,shape", [("D3Q19", (16, 14, 40)), ("D2Q9", (30, 70))])
+@pytest.mark.parametrize("storage", ["f32dev", "f16dev", "f64"])
+def test_halfway_link_words_match_mask_path_bitwise(lattice, shape, storage):
+    """lbm_step's pull sequence reads per-cell link words built once per call;
+    lbm_pull_phase reads the wall mask bytes directly. The same launches in
+    either form give the same bits."""
+    import ctypes
+    from maddening_amd import _lib
+    from maddening_amd.lbm import _ptr, _stream
+    D = len(shape)
+    rng = np.random.default_rng(7)
+    mask = torch.from_numpy(_halfway_walls(shape, 9)).cuda()
+    lid = (0.04, 0.01, -0.03)[:D] + (0.0,) * (3 - D)
+    k = kernel(shape, lattice, 0.7, storage=storage, scheme="pull", bb="halfway", lid_velocity=lid)
+    Q = k.Q
+    e, w, _ = orc.lattice(Q)
+    f0 = orc.equilibrium(1 + 0.01 * rng.standard_normal(shape),
+                         0.03 * rng.standard_normal(shape + (D,)), e, w)
+    f = k.from_aos(torch.from_numpy(f0).cuda())
+    force = [2e-5, -1e-5, 3e-5][:D]
+    nsteps = 6
+    f_step, rho_s, u_s, _ = k.step(f, nsteps, mask=mask, force=force)
+    L = _lib.lib()
+    st = _stream(k.device)
+    d = k.desc(force=force)
+    a, b = k.empty(), k.empty()
+    rho = torch.empty(k.grid_shape, dtype=torch.float32, device="cuda")
+    u = torch.empty(k.grid_shape + (D,), dtype=torch.float32, device="cuda")
+    _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 0, _ptr(f), _ptr(a), _ptr(mask), None, None,
+                                None, None, None), "phase 0")
+    for _ in range(nsteps - 1):
+        _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 1, _ptr(a), _ptr(b), _ptr(mask), None,
+                                    None, None, None, None), "phase 1")
+        a, b = b, a
+    out = k.empty()
+    _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 2, _ptr(a), _ptr(out), _ptr(mask), None,
+                                None, _ptr(rho), _ptr(u), None), "phase 2")
+    torch.cuda.synchronize()
+    assert torch.equal(f_step.view(torch.uint8), out.view(torch.uint8))
+    assert torch.equal(rho_s, rho) and torch.equal(u_s, u)
