@@ -81,7 +81,10 @@ typedef struct lbm_desc {
                            (1, 2 or 4; the contiguous axis must be a multiple of 64*row_vec);
                            0 = auto, -1 = never */
   int32_t flags;        /* LBM_FLAG_* */
-  int32_t reserved;     /* must be 0 */
+  int32_t xcd_group;    /* with LBM_FLAG_XCD_REMAP: row-blocks per run dealt to one XCD
+                           (1 = identity order, 0 = one contiguous eighth of the lattice
+                           per XCD); without the flag the library picks: runs of 512 for
+                           lattices of up to 24576 row-blocks, identity above */
   /* Zou-He pressure faces (lbm.py:425-524, 950-968), applied after streaming,
    * inlet first: 0 = none, 1..6 = x_min, x_max, y_min, y_max, z_min, z_max
    * in the reference's axis order (_FACE_MAP, lbm.py:517-524). */
@@ -92,7 +95,7 @@ typedef struct lbm_desc {
 } lbm_desc;
 
 /* flags */
-#define LBM_FLAG_XCD_REMAP 1  /* deal consecutive rows to one XCD (off by default: measured slower) */
+#define LBM_FLAG_XCD_REMAP 1  /* use xcd_group as given instead of the library's choice */
 /* Adjoint cotangent convention under LBM_BB_REFERENCE (ignored for
  * LBM_BB_NONE, rejected for LBM_BB_HALFWAY): the cotangent buffer holds the
  * slots q and opp(q) of every wall cell exchanged. In that form the transpose

@@ -43,7 +43,7 @@ class LbmDesc(ctypes.Structure):
         ("block_x", ctypes.c_int32),
         ("row_vec", ctypes.c_int32),
         ("flags", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("xcd_group", ctypes.c_int32),
         ("inlet_face", ctypes.c_int32),
         ("outlet_face", ctypes.c_int32),
         ("inlet_rho", ctypes.c_double),

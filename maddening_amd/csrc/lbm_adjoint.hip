@@ -37,7 +37,7 @@ struct AdjArgs {
   const float* p_bar;
   long long plane;
   int na, nb, nc;
-  int bpr, nblocks, remap;
+  int bpr, nblocks, remap;  // remap: xcd_remap run length G (0: identity)
   int out_swapped;      // write fbar_out with the slots of wall cells exchanged
   int in_swapped;       // fbar_in of the moments adjoint is in that form
   int out_faces;        // apply the transpose of the Zou-He faces to fbar_out
@@ -56,23 +56,36 @@ struct AdjArgs {
 // of a uniform force, slots 3-4 dL/drho of the inlet and outlet faces.
 constexpr int kPart = 8;
 
-// Sum NV values over the workgroup and add them to partial[kPart*lb + k].
+// Sum NV values over the workgroup and add them to partial[kPart*lb + k]:
+// an xor butterfly inside each 64-lane wave, then one barrier and a fixed-
+// order sum of the NT/64 wave totals. The order is the same on every launch,
+// so the partials are deterministic.
 template <int NT, int NV>
 __device__ __forceinline__ void block_accumulate(const double (&v)[NV], double* partial, int lb) {
-  __shared__ double sh[NV][NT];
+  constexpr int NW = NT / 64;
+  __shared__ double sh[NV][NW];
+  double s[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int w = NT / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
+  for (int k = 0; k < NV; ++k) s[k] = v[k];
 #pragma unroll
-      for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-    }
-    __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] += __shfl_xor(s[k], off, 64);
   }
+  const int wave = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k][wave] = s[k];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) partial[kPart * (long long)lb + k] += sh[k][0];
+    for (int k = 0; k < NV; ++k) {
+      double t = sh[k][0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) t += sh[k][w];
+      partial[kPart * (long long)lb + k] += t;
+    }
   }
 }
 
@@ -130,7 +143,7 @@ __device__ __forceinline__ void read_force(const AdjArgs<C>& a, int cell, C& Fa,
 template <class L, class T, int BB, bool FPOST, bool GSW, bool FACES>
 __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C> a) {
   using C = typename Storage<T>::C;
-  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   const int row = lb / a.bpr;
   const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
   const bool valid = ic < a.nc;
@@ -146,12 +159,17 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
     if (BB == BB_HALF && self) {
       // a solid cell is reset to rest every step: nothing flows back into it
 #pragma unroll
-      for (int q = 0; q < L::Q; ++q) a.gout[q * P + n.cell] = C(0);
+      for (int q = 0; q < L::Q; ++q) __builtin_nontemporal_store(C(0), a.gout + q * P + n.cell);
     } else {
+      // (cotangents are written with non-temporal stores, as the forward
+      // writes its states: each is read once, by the next transposed step)
       // g_bar = S^T B^T f'_bar: the value the forward step moved out of
       // slot p at this cell went to x = cell + e_p (slot p, or slot opp(p) if
       // x is a wall under the reference rule); under halfway bounce-back a
       // population blocked by a wall came back to this cell in slot opp(p).
+      // dL/dtau so far, read before any store (gout may alias it as far as
+      // the compiler knows, so it could not move this load up by itself)
+      const float tau_bar_in = a.tau_bar ? a.tau_bar[n.cell] : 0.0f;
       C gb[L::Q];
 #pragma unroll
       for (int p = 0; p < L::Q; ++p) {
@@ -247,16 +265,16 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
         rb0 = double(rbf[0]);
         rb1 = double(rbf[1]);
 #pragma unroll
-        for (int p = 0; p < L::Q; ++p) a.gout[p * P + n.cell] = o[p];
+        for (int p = 0; p < L::Q; ++p) __builtin_nontemporal_store(o[p], a.gout + p * P + n.cell);
       } else {
 #pragma unroll
         for (int p = 0; p < L::Q; ++p) {
           const C eB = C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
           const C g = xs ? gb[L::opp[p]] : gb[p];
-          a.gout[p * P + n.cell] = keep * g + A + (xs ? -eB : eB);
+          __builtin_nontemporal_store(keep * g + A + (xs ? -eB : eB), a.gout + p * P + n.cell);
         }
       }
-      if (a.tau_bar) a.tau_bar[n.cell] += float(-om * om * tb);
+      if (a.tau_bar) __builtin_nontemporal_store(tau_bar_in + float(-om * om * tb), a.tau_bar + n.cell);
       // dL/dF: the source directly (u fixed) plus through u = (j + F/2)/rho
       dFa = c * (C(3) * Ga - C(3) * ua * G0 + C(9) * Ka) + C(0.5) * Ba;
       dFb = c * (C(3) * Gb - C(3) * ub * G0 + C(9) * Kb) + C(0.5) * Bb;
@@ -327,12 +345,13 @@ __global__ void __launch_bounds__(256) k_moments_adjoint(AdjArgs<typename Storag
       rb0 = double(rbf[0]);
       rb1 = double(rbf[1]);
 #pragma unroll
-      for (int p = 0; p < L::Q; ++p) a.gout[p * P + cell] = o[p];
+      for (int p = 0; p < L::Q; ++p) __builtin_nontemporal_store(o[p], a.gout + p * P + cell);
     } else {
 #pragma unroll
       for (int p = 0; p < L::Q; ++p) {
         const C base = a.gin ? a.gin[(cross ? L::opp[p] : p) * P + cell] : C(0);
-        a.gout[p * P + cell] = base + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
+        __builtin_nontemporal_store(base + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc),
+                                    a.gout + p * P + cell);
       }
     }
     dFa = C(0.5) * Ba;
@@ -416,7 +435,7 @@ AdjArgs<C> adj_args(const lbm_desc* d, const Geom& g) {
   a.na = g.na; a.nb = g.nb; a.nc = g.nc;
   a.bpr = g.bpr;
   a.nblocks = g.nblocks;
-  a.remap = (d->flags & LBM_FLAG_XCD_REMAP) ? 1 : 0;
+  a.remap = xcd_group(d, g);
   // the exchanged form only means something where walls swap slots
   a.in_swapped = (d->bb_mode == LBM_BB_REFERENCE && (d->flags & LBM_FLAG_ADJ_IN_SWAPPED)) ? 1 : 0;
   a.out_swapped = (d->bb_mode == LBM_BB_REFERENCE && (d->flags & LBM_FLAG_ADJ_OUT_SWAPPED)) ? 1 : 0;

@@ -27,7 +27,7 @@ struct FwdArgs {
   int na, nb, nc;
   int bpr;          // workgroups per grid row
   int nblocks;
-  int remap;
+  int remap;        // xcd_remap run length G (0: identity)
   C inv_tau;
   C Fa, Fb, Fc;
   C lid_a, lid_b, lid_c;
@@ -312,7 +312,7 @@ __device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C
 template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
 __global__ void __launch_bounds__(256)
     k_cell(FwdArgs<typename Storage<T>::C> a) {
-  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   const int row = lb / a.bpr;
   const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
   if (ic >= a.nc) return;

@@ -287,14 +287,19 @@ struct Nbr {
   }
 };
 
-// Map a linear workgroup id to a logical row-block id so that consecutive
-// logical blocks land on the same XCD (workgroups are dealt to the 8 XCDs
-// round-robin; neighbouring rows then share that XCD's L2).
-__device__ __forceinline__ int xcd_remap(int b, int nblocks, bool enable) {
-  if (!enable) return b;
-  const int n8 = (nblocks / 8) * 8;
-  if (b >= n8) return b;
-  return (b % 8) * (n8 / 8) + b / 8;
+// Map a linear workgroup id to a logical row-block id so that runs of G
+// consecutive logical blocks land on one XCD. Workgroups are dealt to the 8
+// XCDs round-robin, so physical block b runs on XCD b % 8; the 8 XCDs then
+// work on 8 adjacent runs at a time and neighbouring rows share an L2.
+// G <= 0 is the identity; G = nblocks / 8 gives each XCD one contiguous
+// eighth of the lattice. Blocks past the last whole set of 8 runs keep
+// their id.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks, int G) {
+  if (G <= 0) return b;
+  const int span = 8 * G;
+  if (b >= (nblocks / span) * span) return b;
+  const int slot = b >> 3;
+  return (slot / G) * span + (b & 7) * G + slot % G;
 }
 
 }  // namespace mlbm

@@ -38,7 +38,7 @@ template <class L, class T, int BB>
 __global__ void __launch_bounds__(256) k_push(FwdArgs<typename Storage<T>::C> a) {
   using C = typename Storage<T>::C;
   using St = Storage<T>;
-  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap != 0);
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   const int row = lb / a.bpr;
   const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
   if (ic >= a.nc) return;
@@ -426,7 +426,7 @@ FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
   a.na = g.na; a.nb = g.nb; a.nc = g.nc;
   a.bpr = g.bpr;
   a.nblocks = g.nblocks;
-  a.remap = (d->flags & LBM_FLAG_XCD_REMAP) ? 1 : 0;
+  a.remap = xcd_group(d, g);
   a.inv_tau = C(1.0 / d->tau);
   if (d->q == 19) {
     a.Fa = C(d->force[0]); a.Fb = C(d->force[1]); a.Fc = C(d->force[2]);

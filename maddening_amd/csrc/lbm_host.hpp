@@ -27,6 +27,25 @@ struct Geom {
   int rowv = 0;  // vector width of the row-per-wave kernel, 0 = per-cell kernel
 };
 
+// Lattices of up to this many row-blocks run faster with runs of
+// kAutoXcdGroup consecutive row-blocks per XCD. Measured on MI355X, D3Q19
+// f32 with 256-thread blocks, per fused launch in TB/s, identity -> runs of 512:
+// 8192 blocks 4.38 -> 5.30, 16384 4.89 -> 5.62, 20480 5.23 -> 5.57,
+// 24576 5.49 -> 5.54, 28672 5.65 -> 5.51, 65536 (256^3) 5.71 -> 5.16;
+// D2Q9 2048^2 (16384 blocks) 4.51 -> 5.48.
+constexpr int kAutoXcdMaxBlocks = 24576;
+constexpr int kAutoXcdGroup = 512;
+
+// Run length G of the XCD remap for a launch over g.nblocks row-blocks
+// (0: identity). Without LBM_FLAG_XCD_REMAP the measured policy above
+// applies; with it, d->xcd_group (1 is the identity, 0 one contiguous eighth
+// of the lattice per XCD).
+inline int xcd_group(const lbm_desc* d, const Geom& g) {
+  if (!(d->flags & LBM_FLAG_XCD_REMAP))
+    return g.nblocks <= kAutoXcdMaxBlocks ? kAutoXcdGroup : 0;
+  return d->xcd_group > 0 ? d->xcd_group : g.nblocks / 8;
+}
+
 inline int storage_size(int s) {
   switch (s) {
     case LBM_F32: return 4;

@@ -55,6 +55,7 @@ struct PipeArgs {
   T* tracer;
   long long plane;
   int na, nb, nc, bpr;
+  int nblocks, remap;  // remap: xcd_remap run length (0: identity)
   int gsrc_aligned;   // tracer_f_{k-1} is the input itself (k = 1)
   T tau, tau_tracer, strength, gravity;
   T G, rho_0, psi_wall, rho_liquid, rho_gas;
@@ -76,8 +77,9 @@ __device__ __forceinline__ T feq(int q, T rho, T ua, T ub, T uc, T usq) {
 template <class T>
 __device__ __forceinline__ void cell_index(const PipeArgs<T>& a, int& ia, int& ib, int& ic,
                                            bool& valid) {
-  const int row = blockIdx.x / a.bpr;
-  ic = (blockIdx.x - row * a.bpr) * blockDim.x + threadIdx.x;
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
+  const int row = lb / a.bpr;
+  ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
   ia = row / a.nb;
   ib = row - ia * a.nb;
   valid = ic < a.nc;
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(256) k_pipe_mp_out(PipeArgs<T> a) {
 // ---- host -------------------------------------------------------------------
 struct PipeGeom {
   long long plane;
-  int na, nb, nc, bpr, nblocks;
+  int na, nb, nc, bx, bpr, nblocks;
 };
 
 inline int pipe_geom(const lbm_pipe_desc* d, PipeGeom* g) {
@@ -396,7 +398,9 @@ inline int pipe_geom(const lbm_pipe_desc* d, PipeGeom* g) {
   g->na = (int)d->dims[0];
   g->nb = (int)d->dims[1];
   g->nc = (int)d->dims[2];
-  g->bpr = (g->nc + 255) / 256;
+  // one thread per cell along the contiguous axis, as the fluid kernels
+  g->bx = g->nc >= 256 ? 256 : ((g->nc + 63) / 64) * 64;
+  g->bpr = (g->nc + g->bx - 1) / g->bx;
   g->nblocks = g->na * g->nb * g->bpr;
   return LBM_OK;
 }
@@ -420,6 +424,8 @@ int pipe_run(hipStream_t s, const lbm_pipe_desc* d, const PipeGeom& g, const T* 
   a.nb = g.nb;
   a.nc = g.nc;
   a.bpr = g.bpr;
+  a.nblocks = g.nblocks;
+  a.remap = g.nblocks <= kAutoXcdMaxBlocks ? kAutoXcdGroup : 0;
   a.tau = T(d->tau);
   a.tau_tracer = T(d->tau_tracer);
   a.strength = T(d->propeller_strength);
@@ -430,7 +436,7 @@ int pipe_run(hipStream_t s, const lbm_pipe_desc* d, const PipeGeom& g, const T* 
   a.rho_liquid = T(d->rho_liquid);
   a.rho_gas = T(d->rho_gas);
   const long long P = g.plane;
-  const dim3 grid(g.nblocks), block(256);
+  const dim3 grid(g.nblocks), block(g.bx);
   if (!d->multiphase) {
     T* F[2] = {work, work + 19 * P};
     T* Gb[2] = {work + 38 * P, work + 45 * P};

@@ -51,8 +51,8 @@ class LBMKernel:
     bb : "none" (periodic), "reference" (lbm.py:944-948) or "halfway".
     lid_velocity : wall velocity of mask==2 cells under "halfway".
     scheme : "auto", "pull" or "push".
-    block_x, row_vec, xcd_remap : launch shape (see include/lbm_c_api.h);
-        0 / 0 / False pick the measured defaults.
+    block_x, row_vec, xcd_remap, xcd_group : launch shape (see
+        include/lbm_c_api.h); 0 / 0 / False / 0 pick the measured defaults.
     """
 
     def __init__(self, grid_shape: Sequence[int], lattice: str = "D3Q19",
@@ -60,7 +60,7 @@ class LBMKernel:
                  storage: str = "f32", bb: str = "none",
                  lid_velocity: Sequence[float] = (0.0, 0.0, 0.0),
                  scheme: str = "auto", block_x: int = 0, row_vec: int = 0,
-                 xcd_remap: bool = False,
+                 xcd_remap: bool = False, xcd_group: int = 0,
                  device="cuda"):
         if lattice.upper() == "D3Q19":
             self.lat: LatticeDescriptor = d3q19()
@@ -93,6 +93,7 @@ class LBMKernel:
         self.scheme = scheme
         self.block_x = int(block_x)
         self.xcd_remap = bool(xcd_remap)
+        self.xcd_group = int(xcd_group)
         self.row_vec = int(row_vec)
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
@@ -127,6 +128,7 @@ class LBMKernel:
         d.block_x = self.block_x
         d.row_vec = self.row_vec
         d.flags = _lib.LBM_FLAG_XCD_REMAP if self.xcd_remap else 0
+        d.xcd_group = self.xcd_group
         if faces is not None:
             for slot, spec in zip(("inlet", "outlet"), faces):
                 if spec is None:
