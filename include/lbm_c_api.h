@@ -76,15 +76,20 @@ typedef struct lbm_desc {
   double tau;           /* BGK relaxation time used where tau_field is NULL; must be > 0.5 */
   double force[3];      /* uniform body force used where force_field is NULL */
   double lid_velocity[3]; /* velocity of mask==2 cells in LBM_BB_HALFWAY */
-  int32_t block_x;      /* threads per workgroup of the one-cell-per-thread kernels; 0 = auto */
+  int32_t block_x;      /* threads per workgroup of the one-cell-per-thread kernels (64, 128,
+                           192 or 256); -1 = the pull step over 256-cell tiles of the
+                           flattened grid; 0 = auto (row blocks of 256 when the contiguous
+                           axis is a multiple of 256 cells, flattened tiles otherwise) */
   int32_t row_vec;      /* periodic lattices only: cells per lane of the row-per-wave kernels
                            (1, 2 or 4; the contiguous axis must be a multiple of 64*row_vec);
                            0 = auto, -1 = never */
   int32_t flags;        /* LBM_FLAG_* */
   int32_t xcd_group;    /* with LBM_FLAG_XCD_REMAP: row-blocks per run dealt to one XCD
                            (1 = identity order, 0 = one contiguous eighth of the lattice
-                           per XCD); without the flag the library picks: runs of 512 for
-                           lattices of up to 24576 row-blocks, identity above */
+                           per XCD); without the flag the library picks: runs of 512 when
+                           the lattice has at most 24576 row-blocks and dims[1]*dims[2]
+                           cells per a-plane (D2Q9: dims[0]*dims[1]) is a power of two,
+                           identity otherwise */
   /* Zou-He pressure faces (lbm.py:425-524, 950-968), applied after streaming,
    * inlet first: 0 = none, 1..6 = x_min, x_max, y_min, y_max, z_min, z_max
    * in the reference's axis order (_FACE_MAP, lbm.py:517-524). */

@@ -144,9 +144,20 @@ template <class L, class T, int BB, bool FPOST, bool GSW, bool FACES>
 __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C> a) {
   using C = typename Storage<T>::C;
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
-  const int row = lb / a.bpr;
-  const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
-  const bool valid = ic < a.nc;
+  int row, ic;
+  bool valid;
+  if (a.bpr == 0) {
+    // 256-cell tiles of the flattened lattice (rows that row blocks would
+    // leave misaligned or ragged, as in the forward)
+    const int cell = lb * 256 + (int)threadIdx.x;
+    valid = cell < (int)a.plane;
+    row = (int)((unsigned)cell / (unsigned)a.nc);
+    ic = cell - row * a.nc;
+  } else {
+    row = lb / a.bpr;
+    ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
+    valid = ic < a.nc;
+  }
   C dFa = 0, dFb = 0, dFc = 0;
   double rb0 = 0, rb1 = 0;
   if (valid) {
@@ -420,6 +431,11 @@ inline int adj_geom(const lbm_desc* d, Geom* g) {
   g->bx = 256;
   g->bpr = (g->nc + 255) / 256;
   g->nblocks = g->na * g->nb * g->bpr;
+  // bpr 0 marks a launch over 256-cell tiles of the flattened lattice
+  if (const int tiles = flat_tiles(d, *g)) {
+    g->bpr = 0;
+    g->nblocks = tiles;
+  }
   return LBM_OK;
 }
 
@@ -435,7 +451,8 @@ AdjArgs<C> adj_args(const lbm_desc* d, const Geom& g) {
   a.na = g.na; a.nb = g.nb; a.nc = g.nc;
   a.bpr = g.bpr;
   a.nblocks = g.nblocks;
-  a.remap = xcd_group(d, g);
+  // flattened tiles keep identity order unless a run length is asked for
+  a.remap = (g.bpr == 0 && !(d->flags & LBM_FLAG_XCD_REMAP)) ? 0 : xcd_group(d, g);
   // the exchanged form only means something where walls swap slots
   a.in_swapped = (d->bb_mode == LBM_BB_REFERENCE && (d->flags & LBM_FLAG_ADJ_IN_SWAPPED)) ? 1 : 0;
   a.out_swapped = (d->bb_mode == LBM_BB_REFERENCE && (d->flags & LBM_FLAG_ADJ_OUT_SWAPPED)) ? 1 : 0;

@@ -28,6 +28,10 @@ struct FwdArgs {
   int bpr;          // workgroups per grid row
   int nblocks;
   int remap;        // xcd_remap run length G (0: identity)
+  // > 0: k_cell runs over flat_blocks tiles of 256 consecutive cells of the
+  // flattened lattice instead of row blocks (flat_remap: their XCD runs)
+  int flat_blocks;
+  int flat_remap;
   C inv_tau;
   C Fa, Fb, Fc;
   C lid_a, lid_b, lid_c;
@@ -312,6 +316,16 @@ __device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C
 template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
 __global__ void __launch_bounds__(256)
     k_cell(FwdArgs<typename Storage<T>::C> a) {
+  if (a.flat_blocks > 0) {
+    // 256-thread tiles of the flattened lattice: every store of a wave is
+    // one aligned 256-byte run whenever the plane size is a multiple of 64
+    const int lb = xcd_remap(blockIdx.x, a.flat_blocks, a.flat_remap);
+    const int cell = lb * 256 + (int)threadIdx.x;
+    if (cell >= (int)a.plane) return;
+    const int row = (int)((unsigned)cell / (unsigned)a.nc);
+    cell_update<L, T, BB, PULL, COLLIDE, STORE>(a, row, cell - row * a.nc);
+    return;
+  }
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   const int row = lb / a.bpr;
   const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;

@@ -427,6 +427,9 @@ FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
   a.bpr = g.bpr;
   a.nblocks = g.nblocks;
   a.remap = xcd_group(d, g);
+  a.flat_blocks = flat_tiles(d, g);
+  if (a.flat_blocks && (d->flags & LBM_FLAG_XCD_REMAP))
+    a.flat_remap = d->xcd_group > 0 ? d->xcd_group : a.flat_blocks / 8;
   a.inv_tau = C(1.0 / d->tau);
   if (d->q == 19) {
     a.Fa = C(d->force[0]); a.Fb = C(d->force[1]); a.Fc = C(d->force[2]);
@@ -486,7 +489,12 @@ int launch_cell(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>:
     if (g.rowv == 2) return launch_row<L, T, 2, PULL, COLLIDE>(s, g, a);
     if (g.rowv == 1) return launch_row<L, T, 1, PULL, COLLIDE>(s, g, a);
   }
-  hipLaunchKernelGGL((k_cell<L, T, BB, PULL, COLLIDE, STORE>), dim3(g.nblocks), dim3(g.bx), 0, s, a);
+  if (a.flat_blocks > 0)
+    hipLaunchKernelGGL((k_cell<L, T, BB, PULL, COLLIDE, STORE>), dim3(a.flat_blocks), dim3(256), 0,
+                       s, a);
+  else
+    hipLaunchKernelGGL((k_cell<L, T, BB, PULL, COLLIDE, STORE>), dim3(g.nblocks), dim3(g.bx), 0, s,
+                       a);
   return check_launch("k_cell launch");
 }
 

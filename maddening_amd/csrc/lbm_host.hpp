@@ -27,23 +27,47 @@ struct Geom {
   int rowv = 0;  // vector width of the row-per-wave kernel, 0 = per-cell kernel
 };
 
-// Lattices of up to this many row-blocks run faster with runs of
-// kAutoXcdGroup consecutive row-blocks per XCD. Measured on MI355X, D3Q19
-// f32 with 256-thread blocks, per fused launch in TB/s, identity -> runs of 512:
-// 8192 blocks 4.38 -> 5.30, 16384 4.89 -> 5.62, 20480 5.23 -> 5.57,
-// 24576 5.49 -> 5.54, 28672 5.65 -> 5.51, 65536 (256^3) 5.71 -> 5.16;
-// D2Q9 2048^2 (16384 blocks) 4.51 -> 5.48.
+// XCD runs (xcd_remap) for row-block launches. Measured on MI355X, D3Q19
+// f32, per fused launch in TB/s, identity -> runs of 512 row-blocks per XCD:
+// lattices whose a-plane (nb * nc cells) is a power of two gain while the
+// grid is small: 64x128x256 4.38 -> 5.30, 128x128x256 4.89 -> 5.62,
+// 160x128x256 5.23 -> 5.57, 128^3 4.53 -> 5.05, D2Q9 2048^2 4.51 -> 5.48;
+// above 24576 row-blocks they lose (224x128x256 5.65 -> 5.51, 256^3 5.71 ->
+// 5.16). Other shapes lose at any size: 100x100x256 5.31 -> 5.19,
+// 128x128x192 5.22 -> 4.93, 64x300x256 5.48 -> 5.27. Shorter runs (2-64)
+// stay within a few per cent of identity everywhere.
 constexpr int kAutoXcdMaxBlocks = 24576;
 constexpr int kAutoXcdGroup = 512;
 
-// Run length G of the XCD remap for a launch over g.nblocks row-blocks
-// (0: identity). Without LBM_FLAG_XCD_REMAP the measured policy above
-// applies; with it, d->xcd_group (1 is the identity, 0 one contiguous eighth
-// of the lattice per XCD).
+inline bool pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Run length G of the XCD remap for a row-block launch over g (0: identity).
+// Without LBM_FLAG_XCD_REMAP the measured policy above applies; with it,
+// d->xcd_group (1 is the identity, 0 one contiguous eighth of the lattice
+// per XCD).
 inline int xcd_group(const lbm_desc* d, const Geom& g) {
   if (!(d->flags & LBM_FLAG_XCD_REMAP))
-    return g.nblocks <= kAutoXcdMaxBlocks ? kAutoXcdGroup : 0;
+    return (g.nblocks <= kAutoXcdMaxBlocks && pow2((long long)g.nb * g.nc)) ? kAutoXcdGroup : 0;
   return d->xcd_group > 0 ? d->xcd_group : g.nblocks / 8;
+}
+
+// Row blocks leave rows that are not a multiple of 64 cells misaligned, and
+// rows that are not a multiple of 256 cells end in a partly idle workgroup
+// (or use workgroups of 1-3 waves). Measured on MI355X, D3Q19 f32 per fused
+// launch, row blocks -> 256-cell tiles of the flattened grid, TB/s:
+// 256x160x161 3.06 -> 5.41, 256x160x193 3.36 -> 5.50, 321x256x255 3.71 ->
+// 5.64, 100^3 3.78 -> 4.99, 300x300x64 5.25 -> 5.65, 256x256x128 5.37 ->
+// 5.56, 150x150x64 4.89 -> 5.09; rows of 192 come out even (5.27 -> 5.11,
+// 4.89 -> 4.86, 5.23 -> 5.25) and rows of 256 too (5.64 -> 5.63). A plane
+// size that is not a multiple of 64 keeps most stores misaligned either way
+// (161^3 3.12 -> 3.48). So row blocks are kept for rows that are a multiple
+// of 256 cells, and everything else runs over flattened tiles, in identity
+// order: runs of 512 there went from +9 % (128x128x192) to -23 % (160^3).
+// block_x > 0 asks for row blocks of that width, -1 for flattened tiles.
+inline int flat_tiles(const lbm_desc* d, const Geom& g) {
+  if (d->block_x > 0) return 0;
+  if (d->block_x == 0 && g.nc % 256 == 0) return 0;
+  return (int)((g.plane + 255) / 256);
 }
 
 inline int storage_size(int s) {
@@ -81,11 +105,11 @@ inline int check_desc(const lbm_desc* d, Geom* g) {
     return fail(LBM_ERR_UNSUPPORTED, "more than 2^31-1 cells per population plane");
   g->plane = plane;
   int bx = d->block_x;
-  if (bx == 0) {
+  if (bx == 0 || bx == -1) {
     bx = g->nc >= 256 ? 256 : ((g->nc + 63) / 64) * 64;
   }
   if (bx < 64 || bx > 256 || bx % 64 != 0)
-    return fail(LBM_ERR_INVALID, "block_x must be 64, 128, 192 or 256");
+    return fail(LBM_ERR_INVALID, "block_x must be 0, -1, 64, 128, 192 or 256");
   if (d->row_vec != 0 && d->row_vec != -1 && d->row_vec != 1 && d->row_vec != 2 && d->row_vec != 4)
     return fail(LBM_ERR_INVALID, "row_vec must be -1, 0, 1, 2 or 4");
   if (d->row_vec == 4 && d->storage == LBM_F64)

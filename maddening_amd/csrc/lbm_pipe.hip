@@ -78,11 +78,20 @@ template <class T>
 __device__ __forceinline__ void cell_index(const PipeArgs<T>& a, int& ia, int& ib, int& ic,
                                            bool& valid) {
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
-  const int row = lb / a.bpr;
-  ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
+  int row;
+  if (a.bpr == 0) {
+    // 256-cell tiles of the flattened lattice
+    const int cell = lb * 256 + (int)threadIdx.x;
+    valid = cell < (int)a.plane;
+    row = (int)((unsigned)cell / (unsigned)a.nc);
+    ic = cell - row * a.nc;
+  } else {
+    row = lb / a.bpr;
+    ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
+    valid = ic < a.nc;
+  }
   ia = row / a.nb;
   ib = row - ia * a.nb;
-  valid = ic < a.nc;
 }
 
 // ---- single phase -----------------------------------------------------------
@@ -398,10 +407,16 @@ inline int pipe_geom(const lbm_pipe_desc* d, PipeGeom* g) {
   g->na = (int)d->dims[0];
   g->nb = (int)d->dims[1];
   g->nc = (int)d->dims[2];
-  // one thread per cell along the contiguous axis, as the fluid kernels
-  g->bx = g->nc >= 256 ? 256 : ((g->nc + 63) / 64) * 64;
-  g->bpr = (g->nc + g->bx - 1) / g->bx;
-  g->nblocks = g->na * g->nb * g->bpr;
+  // as the fluid kernels: row blocks of 256 threads when rows are a multiple
+  // of 256 cells, else 256-cell tiles of the flattened grid (bpr 0)
+  g->bx = 256;
+  if (g->nc % 256 == 0) {
+    g->bpr = g->nc / 256;
+    g->nblocks = g->na * g->nb * g->bpr;
+  } else {
+    g->bpr = 0;
+    g->nblocks = (int)((plane + 255) / 256);
+  }
   return LBM_OK;
 }
 
@@ -425,7 +440,8 @@ int pipe_run(hipStream_t s, const lbm_pipe_desc* d, const PipeGeom& g, const T* 
   a.nc = g.nc;
   a.bpr = g.bpr;
   a.nblocks = g.nblocks;
-  a.remap = g.nblocks <= kAutoXcdMaxBlocks ? kAutoXcdGroup : 0;
+  a.remap = (g.bpr > 0 && g.nblocks <= kAutoXcdMaxBlocks && pow2((long long)g.nb * g.nc))
+                ? kAutoXcdGroup : 0;
   a.tau = T(d->tau);
   a.tau_tracer = T(d->tau_tracer);
   a.strength = T(d->propeller_strength);
