@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LBM_API_VERSION 4
+#define LBM_API_VERSION 5
 
 typedef enum lbm_status {
   LBM_OK = 0,
@@ -219,6 +219,9 @@ typedef struct lbm_batch_item {
   float* rho;             /* nullable moment outputs written by the last launch of a run */
   float* u;
   float* p;
+  int64_t pitch;          /* cells from the start of one population plane of f[0]/f[1] to the
+                             next: 0 = the cell count (the lbm_step layout); else at least the
+                             cell count. A multiple of 64 keeps every store of a step aligned */
 } lbm_batch_item;
 
 /* Opaque host-side plan: the class ranges the launches need. */

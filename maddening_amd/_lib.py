@@ -84,6 +84,7 @@ class LbmBatchItem(ctypes.Structure):
         ("rho", ctypes.c_void_p),
         ("u", ctypes.c_void_p),
         ("p", ctypes.c_void_p),
+        ("pitch", ctypes.c_int64),
     ]
 
 

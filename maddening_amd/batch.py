@@ -5,6 +5,14 @@ device plan of ``lbm_batch_plan``. ``run(nsteps)`` launches the whole batch
 (nsteps + 1 launches); ``graph(nsteps)`` captures those launches once into a
 HIP graph that ``launch_graph`` replays. ``run_per_lattice`` is the
 comparison point: one ``lbm_step`` call per lattice.
+
+The batch buffers pad each population plane to a multiple of
+``plane_align`` cells (64 by default), so that every store of a step is an
+aligned 256-byte run whatever the lattice's cell count. Measured on MI355X,
+a lattice whose cell count is not a multiple of 64 otherwise stores
+misaligned runs and its step moves about a third fewer bytes per second.
+``state_views`` gives the padded buffers as (Q, *G) views; ``states`` and
+``set_states`` use the ordinary lbm_step layout.
 """
 from __future__ import annotations
 
@@ -26,20 +34,29 @@ class LatticeBatch:
     "reference", "halfway"), ``mask`` (uint8 array), ``lid_velocity``.
     """
 
-    def __init__(self, specs: Sequence[dict], *, cells_per_thread: int = 1, device="cuda"):
+    def __init__(self, specs: Sequence[dict], *, cells_per_thread: int = 1, device="cuda",
+                 plane_align: int = 64):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
+        if plane_align < 1:
+            raise ValueError("plane_align must be >= 1")
         self.kernels = []
-        self.buffers = []
+        self.buffers = []      # (Q, *G) views of the padded state buffers
+        self.pitches = []      # cells from one population plane to the next
         self.masks = []
         self.specs = list(specs)
+        self._std = None       # lbm_step-layout buffers of the per-lattice baseline
         for s in self.specs:
             bb = s.get("bb", "none")
             k = LBMKernel(s["shape"], lattice=s["lattice"], tau=s["tau"], storage="f32", bb=bb,
                           lid_velocity=s.get("lid_velocity", (0.0, 0.0, 0.0)), device=self.device)
             self.kernels.append(k)
-            self.buffers.append((k.empty(), k.empty()))
+            pitch = -(-k.ncells // plane_align) * plane_align
+            self.pitches.append(pitch)
+            self.buffers.append(tuple(
+                torch.empty(k.Q * pitch, dtype=torch.float32, device=self.device)
+                .view(k.Q, pitch)[:, :k.ncells].view(k.state_shape) for _ in range(2)))
             m = s.get("mask")
             self.masks.append(None if m is None else
                               torch.as_tensor(np.asarray(m, np.uint8), device=self.device))
@@ -68,6 +85,7 @@ class LatticeBatch:
             it.f[0] = self.buffers[i][0].data_ptr()
             it.f[1] = self.buffers[i][1].data_ptr()
             it.mask = None if self.masks[i] is None else self.masks[i].data_ptr()
+            it.pitch = self.pitches[i]
         return arr
 
     def plan(self, cells_per_thread: int) -> None:
@@ -94,13 +112,22 @@ class LatticeBatch:
 
     # -- state ---------------------------------------------------------------
     def set_states(self, states: Sequence[torch.Tensor]) -> None:
-        """Copy initial SoA f32 states into the first buffer of each lattice."""
-        for (a, _), f in zip(self.buffers, states):
+        """Copy initial SoA f32 states (lbm_step layout) into the first buffer
+        of each lattice, and into the per-lattice baseline's buffers."""
+        for i, ((a, _), f) in enumerate(zip(self.buffers, states)):
             a.copy_(f)
+            if self._std is not None:
+                self._std[i][0].copy_(f)
+
+    def state_views(self, nsteps: int):
+        """(Q, *G) views of the padded buffers holding the final state after a
+        run of ``nsteps`` steps (not contiguous when planes are padded)."""
+        return [bufs[(nsteps + 1) % 2] for bufs in self.buffers]
 
     def states(self, nsteps: int):
-        """Buffers holding the final state after a run of ``nsteps`` steps."""
-        return [bufs[(nsteps + 1) % 2] for bufs in self.buffers]
+        """The final states after a run of ``nsteps`` steps, in the lbm_step
+        layout (copies where planes are padded)."""
+        return [v if v.is_contiguous() else v.contiguous() for v in self.state_views(nsteps)]
 
     # -- execution -------------------------------------------------------------
     def run(self, nsteps: int, grid: Optional[int] = None) -> None:
@@ -122,10 +149,16 @@ class LatticeBatch:
                    "lbm_batch_graph_launch")
 
     def run_per_lattice(self, nsteps: int):
-        """Baseline: one lbm_step call per lattice, from the first buffer into the
-        second (nsteps + 1 launches per lattice). Returns the second buffers."""
+        """Baseline: one lbm_step call per lattice (nsteps + 1 launches each),
+        from the states last given to set_states, in lbm_step-layout buffers
+        of its own (made from the batch's first buffers on the first call).
+        Returns the final states."""
+        if self._std is None:
+            self._std = [(a.clone(memory_format=torch.contiguous_format),
+                          torch.empty_like(a, memory_format=torch.contiguous_format))
+                         for a, _ in self.buffers]
         outs = []
-        for k, (a, b), m, s in zip(self.kernels, self.buffers, self.masks, self.specs):
+        for k, (a, b), m, s in zip(self.kernels, self._std, self.masks, self.specs):
             k.step(a, nsteps, mask=m, force=list(s.get("force", (0.0,) * k.D)), out=b,
                    moments=False)
             outs.append(b)

@@ -29,7 +29,8 @@ namespace mlbm {
 struct BatchEntry {
   int q, bb;
   int na, nb, nc, bpr;
-  long long plane;
+  long long plane;       // cells
+  long long pitch;       // cells from one population plane to the next
   long long tile_begin;
   float inv_tau, Fa, Fb, Fc, lid_a, lid_b, lid_c;
   int pad;
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(256)
     a.src = e.buf[parity];
     a.dst = e.buf[parity ^ 1];
     a.mask = e.mask;
-    a.plane = e.plane;
+    a.plane = e.pitch;  // cell_update uses plane as the stride between planes
     a.na = e.na; a.nb = e.nb; a.nc = e.nc;
     a.inv_tau = e.inv_tau;
     a.Fa = e.Fa; a.Fb = e.Fb; a.Fc = e.Fc;
@@ -204,6 +205,10 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
       return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": needs two distinct state buffers");
     if (it.bb_mode != LBM_BB_NONE && !it.mask)
       return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": bb_mode needs a mask");
+    if (it.pitch != 0 && it.pitch < geo[i].plane)
+      return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": pitch is below the cell count");
+    if (it.pitch > 0x7fffffffLL)
+      return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": pitch must fit int32");
   }
   std::vector<int> order(nitems);
   for (int i = 0; i < nitems; ++i) order[i] = i;
@@ -226,6 +231,7 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     e.na = g.na; e.nb = g.nb; e.nc = g.nc;
     e.bpr = g.bpr;
     e.plane = g.plane;
+    e.pitch = it.pitch ? it.pitch : g.plane;
     e.tile_begin = tiles;
     e.inv_tau = float(1.0 / it.tau);
     if (it.q == 19) {

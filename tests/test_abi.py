@@ -43,7 +43,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_string_without_gpu():
     from maddening_amd import _lib
     L = _lib.lib()
-    assert L.lbm_version() == 4
+    assert L.lbm_version() == 5
     assert isinstance(L.lbm_last_error(), bytes)
 
 
@@ -77,6 +77,23 @@ def test_host_validation_errors_without_gpu():
     assert L.lbm_workspace_size(ctypes.byref(d), 10) == 8 ** 3 * 19 * 4 + 8 ** 3 * 4
     d.storage, d.dims[0], d.dims[1], d.dims[2] = 2, 3, 5, 7
     assert L.lbm_workspace_size(ctypes.byref(d), 10) == 4096 + 105 * 4
+
+
+def test_batch_item_validation_without_gpu():
+    """Batch items are checked before the plan is uploaded: a plane pitch
+    below the cell count is refused."""
+    from maddening_amd import _lib
+    L = _lib.lib()
+    items = (_lib.LbmBatchItem * 1)()
+    it = items[0]
+    it.q, it.bb_mode, it.tau = 19, 0, 0.8
+    it.dims[0], it.dims[1], it.dims[2] = 5, 6, 7
+    it.f[0], it.f[1] = 4096, 8192          # never dereferenced: validation fails first
+    it.pitch = 5 * 6 * 7 - 1
+    handle = ctypes.c_void_p()
+    rc = L.lbm_batch_plan_create(None, items, 1, 1, ctypes.c_void_p(4096), ctypes.byref(handle),
+                                 None)
+    assert rc == -1 and b"pitch is below the cell count" in L.lbm_last_error()
 
 
 def test_desc_layout_matches_c(tmp_path):

@@ -53,10 +53,12 @@ def reference_results(batch, f0s, nsteps):
     return res
 
 
-@pytest.mark.parametrize("cpt", [1, 2, 4])
-def test_batch_equals_single_lattice_steps(cpt):
+@pytest.mark.parametrize("cpt,align", [(1, 64), (2, 64), (4, 64), (1, 1), (2, 37)])
+def test_batch_equals_single_lattice_steps(cpt, align):
+    """Padded planes (align 64 or 37 cells) and the lbm_step layout (align 1)
+    give the single-lattice results bit for bit."""
     from maddening_amd.batch import LatticeBatch
-    batch = LatticeBatch(mixed_specs(), cells_per_thread=cpt)
+    batch = LatticeBatch(mixed_specs(), cells_per_thread=cpt, plane_align=align)
     f0s = initial_states(batch)
     nsteps = 11
     ref = reference_results(batch, f0s, nsteps)
@@ -65,7 +67,7 @@ def test_batch_equals_single_lattice_steps(cpt):
         batch.run(nsteps, grid=grid)
         torch.cuda.synchronize()
         for i, (a, b) in enumerate(zip(batch.states(nsteps), ref)):
-            assert torch.equal(a, b), (cpt, grid, i, float((a - b).abs().max()))
+            assert torch.equal(a, b), (cpt, align, grid, i, float((a - b).abs().max()))
 
 
 def test_batch_graph_replays_match_eager():
