@@ -222,6 +222,10 @@ typedef struct lbm_batch_item {
   int64_t pitch;          /* cells from the start of one population plane of f[0]/f[1] to the
                              next: 0 = the cell count (the lbm_step layout); else at least the
                              cell count. A multiple of 64 keeps every store of a step aligned */
+  uint32_t* links;        /* LBM_BB_HALFWAY only, nullable: 4 bytes per cell of device memory
+                             for link words, rebuilt from the mask at the start of every run;
+                             the steps then read one word per cell instead of the neighbours'
+                             mask bytes (the same results) */
 } lbm_batch_item;
 
 /* Opaque host-side plan: the class ranges the launches need. */

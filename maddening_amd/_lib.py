@@ -85,6 +85,7 @@ class LbmBatchItem(ctypes.Structure):
         ("u", ctypes.c_void_p),
         ("p", ctypes.c_void_p),
         ("pitch", ctypes.c_int64),
+        ("links", ctypes.c_void_p),
     ]
 
 

@@ -12,7 +12,10 @@ aligned 256-byte run whatever the lattice's cell count. Measured on MI355X,
 a lattice whose cell count is not a multiple of 64 otherwise stores
 misaligned runs and its step moves about a third fewer bytes per second.
 ``state_views`` gives the padded buffers as (Q, *G) views; ``states`` and
-``set_states`` use the ordinary lbm_step layout.
+``set_states`` use the ordinary lbm_step layout. Lattices under halfway
+bounce-back also get a buffer of per-cell link words (4 bytes per cell),
+rebuilt from the mask at the start of every run, so that their steps read
+one word per cell instead of the neighbours' mask bytes.
 """
 from __future__ import annotations
 
@@ -35,7 +38,7 @@ class LatticeBatch:
     """
 
     def __init__(self, specs: Sequence[dict], *, cells_per_thread: int = 1, device="cuda",
-                 plane_align: int = 64):
+                 plane_align: int = 64, link_words: bool = True):
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -47,6 +50,7 @@ class LatticeBatch:
         self.masks = []
         self.specs = list(specs)
         self._std = None       # lbm_step-layout buffers of the per-lattice baseline
+        self.links = []        # link-word buffers of halfway lattices (None elsewhere)
         for s in self.specs:
             bb = s.get("bb", "none")
             k = LBMKernel(s["shape"], lattice=s["lattice"], tau=s["tau"], storage="f32", bb=bb,
@@ -57,6 +61,8 @@ class LatticeBatch:
             self.buffers.append(tuple(
                 torch.empty(k.Q * pitch, dtype=torch.float32, device=self.device)
                 .view(k.Q, pitch)[:, :k.ncells].view(k.state_shape) for _ in range(2)))
+            self.links.append(torch.empty(k.ncells, dtype=torch.int32, device=self.device)
+                              if (bb == "halfway" and link_words) else None)
             m = s.get("mask")
             self.masks.append(None if m is None else
                               torch.as_tensor(np.asarray(m, np.uint8), device=self.device))
@@ -86,6 +92,7 @@ class LatticeBatch:
             it.f[1] = self.buffers[i][1].data_ptr()
             it.mask = None if self.masks[i] is None else self.masks[i].data_ptr()
             it.pitch = self.pitches[i]
+            it.links = None if self.links[i] is None else self.links[i].data_ptr()
         return arr
 
     def plan(self, cells_per_thread: int) -> None:

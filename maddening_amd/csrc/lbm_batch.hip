@@ -36,6 +36,7 @@ struct BatchEntry {
   int pad;
   void* buf[2];
   const uint8_t* mask;
+  const uint32_t* links;  // halfway link words, or null (mask path)
   float* rho;
   float* u;
   float* p;
@@ -80,6 +81,7 @@ __global__ void __launch_bounds__(256)
     a.src = e.buf[parity];
     a.dst = e.buf[parity ^ 1];
     a.mask = e.mask;
+    a.links = e.links;
     a.plane = e.pitch;  // cell_update uses plane as the stride between planes
     a.na = e.na; a.nb = e.nb; a.nc = e.nc;
     a.inv_tau = e.inv_tau;
@@ -128,11 +130,22 @@ int group_launch(hipStream_t s, const void* dplan, const BatchGroup& gr, int pha
 
 }  // namespace mlbm
 
+namespace mlbm {
+// A halfway item whose link words are rebuilt at the start of each run.
+struct LinkJob {
+  int q;
+  const uint8_t* mask;
+  uint32_t* links;
+  Geom g;
+};
+}  // namespace mlbm
+
 // Host handle of a plan: the device table plus the class ranges the
 // launches need (host memory; the device table is the caller's buffer).
 struct lbm_batch_plan_t {
   const void* dplan;
   std::vector<mlbm::BatchGroup> groups;
+  std::vector<mlbm::LinkJob> links;
 };
 
 namespace mlbm {
@@ -156,6 +169,15 @@ inline int batch_launch(hipStream_t s, const lbm_batch_plan_t* p, int phase, int
 }
 
 inline int batch_run(hipStream_t s, const lbm_batch_plan_t* p, int nsteps, int grid) {
+  for (const LinkJob& j : p->links) {
+    if (j.q == 19)
+      hipLaunchKernelGGL((k_links<Lattice<19>>), dim3(j.g.nblocks), dim3(j.g.bx), 0, s, j.mask,
+                         j.links, j.g.na, j.g.nb, j.g.nc, j.g.bpr);
+    else
+      hipLaunchKernelGGL((k_links<Lattice<9>>), dim3(j.g.nblocks), dim3(j.g.bx), 0, s, j.mask,
+                         j.links, j.g.na, j.g.nb, j.g.nc, j.g.bpr);
+    if (int rc = check_launch("k_links launch")) return rc;
+  }
   int rc = batch_launch(s, p, 0, 0, grid);
   for (int k = 1; k < nsteps && rc == LBM_OK; ++k) rc = batch_launch(s, p, 1, k & 1, grid);
   if (rc == LBM_OK) rc = batch_launch(s, p, 2, nsteps & 1, grid);
@@ -245,6 +267,11 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     e.buf[0] = it.f[0];
     e.buf[1] = it.f[1];
     e.mask = it.bb_mode == LBM_BB_NONE ? nullptr : it.mask;
+    e.links = nullptr;
+    if (it.bb_mode == LBM_BB_HALFWAY && it.links) {
+      e.links = it.links;
+      hp->links.push_back(LinkJob{it.q, it.mask, it.links, g});
+    }
     e.rho = it.rho;
     e.u = it.u;
     e.p = it.p;

@@ -333,4 +333,31 @@ __global__ void __launch_bounds__(256)
   cell_update<L, T, BB, PULL, COLLIDE, STORE>(a, row, ic);
 }
 
+// Link words for halfway bounce-back (FwdArgs::links), built at the start of
+// each lbm_step call or batch run (the mask is an input of each): reads the
+// cell's mask byte and its neighbours' (from cache), writes 4 bytes per cell.
+template <class L>
+__global__ void __launch_bounds__(256)
+    k_links(const uint8_t* __restrict__ mask, uint32_t* __restrict__ links, int na, int nb, int nc,
+            int bpr) {
+  const int row = blockIdx.x / bpr;
+  const int ic = (blockIdx.x - row * bpr) * blockDim.x + threadIdx.x;
+  if (ic >= nc) return;
+  const int ia = row / nb;
+  const int ib = row - ia * nb;
+  Nbr n;
+  n.init(ia, ib, ic, na, nb, nc);
+  uint32_t w = 0;
+  if (mask[n.cell]) {
+    w = kLinkSolid;
+  } else {
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) {
+      const uint8_t m = mask[n.template at<L>(q, -1)];
+      if (m) w |= (1u << q) | (m == 2 ? kLinkMoving : 0u);
+    }
+  }
+  links[n.cell] = w;
+}
+
 }  // namespace mlbm

@@ -53,12 +53,14 @@ def reference_results(batch, f0s, nsteps):
     return res
 
 
-@pytest.mark.parametrize("cpt,align", [(1, 64), (2, 64), (4, 64), (1, 1), (2, 37)])
-def test_batch_equals_single_lattice_steps(cpt, align):
-    """Padded planes (align 64 or 37 cells) and the lbm_step layout (align 1)
-    give the single-lattice results bit for bit."""
+@pytest.mark.parametrize("cpt,align,links", [(1, 64, True), (2, 64, True), (4, 64, True),
+                                             (1, 1, True), (2, 37, True), (1, 64, False)])
+def test_batch_equals_single_lattice_steps(cpt, align, links):
+    """Padded planes (align 64 or 37 cells) and the lbm_step layout (align 1),
+    with halfway link words or the mask path, give the single-lattice results
+    bit for bit."""
     from maddening_amd.batch import LatticeBatch
-    batch = LatticeBatch(mixed_specs(), cells_per_thread=cpt, plane_align=align)
+    batch = LatticeBatch(mixed_specs(), cells_per_thread=cpt, plane_align=align, link_words=links)
     f0s = initial_states(batch)
     nsteps = 11
     ref = reference_results(batch, f0s, nsteps)
@@ -96,3 +98,24 @@ def test_per_lattice_baseline_matches():
     ref = reference_results(batch, f0s, 5)
     for a, b in zip(outs, ref):
         assert torch.equal(a, b)
+
+
+def test_batch_link_words_follow_mask_updates():
+    """Link words are rebuilt from the mask at the start of every run: a
+    mask changed in place between runs is seen by the next run."""
+    from maddening_amd.batch import LatticeBatch
+    batch = LatticeBatch(mixed_specs(), cells_per_thread=1)
+    f0s = initial_states(batch)
+    nsteps = 4
+    batch.set_states(f0s)
+    batch.run(nsteps)
+    torch.cuda.synchronize()
+    i = 5                                   # the D3Q19 halfway lattice
+    assert batch.specs[i]["bb"] == "halfway"
+    m = batch.masks[i]
+    m[5:8, 2:4, 6:10] = 1                   # new static walls, in place
+    ref = reference_results(batch, f0s, nsteps)
+    batch.set_states(f0s)
+    batch.run(nsteps)
+    torch.cuda.synchronize()
+    assert torch.equal(batch.states(nsteps)[i], ref[i])
