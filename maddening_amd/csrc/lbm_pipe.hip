@@ -157,10 +157,10 @@ __global__ void __launch_bounds__(256) k_pipe(PipeArgs<T> a) {
       }
     }
 #pragma unroll
-    for (int q = 0; q < 7; ++q) a.gdst[q * P + n.cell] = g[q];
+    for (int q = 0; q < 7; ++q) __builtin_nontemporal_store(g[q], a.gdst + q * P + n.cell);
     if (PHASE == 2) {
 #pragma unroll
-      for (int q = 0; q < 19; ++q) a.fdst[q * P + n.cell] = f[q];
+      for (int q = 0; q < 19; ++q) __builtin_nontemporal_store(f[q], a.fdst + q * P + n.cell);
       if (a.rho) a.rho[n.cell] = rho;
       if (a.u) {
         a.u[3 * (long long)n.cell + 0] = oa;
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_pipe(PipeArgs<T> a) {
     const T eF = T(L::ea[q]) * Fa + T(L::ec[q]) * Fc;
     const T fe = feq<T>(q, rho, ua, ub, uc, usq);
     const T S = pref * w19<T>(q) * (T(3) * (eF - uF) + T(9) * eu * eF);
-    a.fdst[q * P + n.cell] = f[q] - (f[q] - fe) / a.tau + S;
+    __builtin_nontemporal_store(f[q] - (f[q] - fe) / a.tau + S, a.fdst + q * P + n.cell);
   }
 }
 
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) k_pipe_tracer(PipeArgs<T> a) {
     const int up = n.row[1 + dq[0]][1 + dq[1]] + n.col[1 + dq[2]];
     const int dn = n.row[1 - dq[0]][1 - dq[1]] + n.col[1 - dq[2]];
     const T g = wall ? a.gsrc[Q7::opp[q] * P + up] : a.gsrc[q * P + dn];
-    a.gdst[q * P + n.cell] = g;
+    __builtin_nontemporal_store(g, a.gdst + q * P + n.cell);
     c += g;
   }
   if (a.tracer) a.tracer[n.cell] = wall ? T(0) : c;
@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(256) k_pipe_mp(PipeArgs<T> a) {
   for (int q = 0; q < 19; ++q) {
     const T fb = feq<T>(q, rho, ua, ub, uc, usq);
     const T fsh = feq<T>(q, rho, sa, sb, sc, ssq);
-    a.fdst[q * P + n.cell] = f[q] - (f[q] - fb) / a.tau + (fsh - fb);
+    __builtin_nontemporal_store(f[q] - (f[q] - fb) / a.tau + (fsh - fb), a.fdst + q * P + n.cell);
   }
 }
 
@@ -367,7 +367,8 @@ __global__ void __launch_bounds__(256) k_pipe_mp_out(PipeArgs<T> a) {
   const T inv = T(1) / (rho > T(1e-10) ? rho : T(1e-10));
   const T s = floor_scale<T>(rho);
 #pragma unroll
-  for (int q = 0; q < 19; ++q) a.fdst[q * P + n.cell] = s != T(1) ? f[q] * s : f[q];
+  for (int q = 0; q < 19; ++q)
+    __builtin_nontemporal_store(s != T(1) ? f[q] * s : f[q], a.fdst + q * P + n.cell);
   const T rho_out = s != T(1) ? T(0.01) : rho;
   if (a.rho) a.rho[n.cell] = rho_out;
   if (a.u) {
