@@ -1,0 +1,165 @@
+"""The benchmark workloads at their full sizes, on the GPU.
+
+The float64 HIP kernel is the large-grid reference: tests/test_gpu_forward.py
+holds it to the float64 NumPy oracle at 1e-12 on small grids, and the oracle
+itself is pinned by tests/test_oracle.py and the golden vectors.
+
+* D3Q19 256^3 Taylor-Green (u0 = 0.05, tau = 0.8), 1000 steps: float32 and
+  float32-deviation storage within 1e-5 relative of float64 on rho and u.
+* The same case with float16 deviation storage: max |du| / u0 below 1e-3
+  (measured 9.3e-4); bfloat16 deviations below 5e-2 (measured 3.3e-2).
+* D3Q19 128x128x256 sphere-packed channel (porosity 0.7, tau field 0.7,
+  F = (0, 0, 1e-5)), 500 steps, loss = mean u_z over the outlet plane:
+  the sqrt(N)-checkpointed float64 adjoint against central differences,
+  dL/dF to 1e-6 relative and dL/dtau(x) on four seeded fluid cells to 1e-4.
+* The 64-lattice uneven batch, 200 steps: the graph-replayed batch equals
+  one lbm_step call per lattice bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from maddening_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _tgv_fields(n, u0):
+    rho, u = workloads.taylor_green_3d(n, u0)
+    return (torch.from_numpy(rho).float().cuda(), torch.from_numpy(u).float().cuda())
+
+
+def _rel(a, b):
+    return float((a.double() - b.double()).abs().max() / b.double().abs().max())
+
+
+@pytest.fixture(scope="module")
+def tgv256_reference():
+    """float64 kernel state after 1000 steps of the 256^3 Taylor-Green case."""
+    from maddening_amd.lbm import LBMKernel
+    rho0, u0 = _tgv_fields(256, 0.05)
+    k = LBMKernel((256,) * 3, tau=0.8, storage="f64", device="cuda")
+    f = k.equilibrium(rho0, u0)
+    m0 = k.conservation(f)
+    f, rho, u, _ = k.step(f, 1000)
+    m1 = k.conservation(f)
+    out = {"rho": rho, "u": u, "mass": (float(m0[0]), float(m1[0])), "fields": (rho0, u0)}
+    del f, k
+    torch.cuda.empty_cache()
+    yield out
+    out.clear()
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32dev"])
+def test_config2_256_float32_within_1e5_after_1000_steps(tgv256_reference, storage):
+    from maddening_amd.lbm import LBMKernel
+    ref = tgv256_reference
+    k = LBMKernel((256,) * 3, tau=0.8, storage=storage, device="cuda")
+    f = k.equilibrium(*ref["fields"])
+    f, rho, u, _ = k.step(f, 1000)
+    torch.cuda.synchronize()
+    assert _rel(rho, ref["rho"]) < 1e-5
+    assert _rel(u, ref["u"]) < 1e-5
+    # the float64 reference conserves mass to round-off over 1000 steps
+    m0, m1 = ref["mass"]
+    assert abs(m1 - m0) / m0 < 1e-12
+    del f, rho, u, k
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("storage,bound", [("f16dev", 1e-3), ("bf16dev", 5e-2)])
+def test_config3_256_deviation_storage_after_1000_steps(tgv256_reference, storage, bound):
+    from maddening_amd.lbm import LBMKernel
+    ref = tgv256_reference
+    k = LBMKernel((256,) * 3, tau=0.8, storage=storage, device="cuda")
+    f = k.equilibrium(*ref["fields"])
+    f, rho, u, _ = k.step(f, 1000)
+    torch.cuda.synchronize()
+    du = float((u.double() - ref["u"].double()).abs().max()) / 0.05
+    assert du < bound, du
+    del f, rho, u, k
+    torch.cuda.empty_cache()
+
+
+def test_config4_full_size_adjoint_matches_central_differences():
+    from maddening_amd.adjoint import rollout
+    from maddening_amd.lbm import LBMKernel
+    shape, nsteps, F0 = (128, 128, 256), 500, 1e-5
+    mask = torch.from_numpy(workloads.sphere_packing(shape, porosity=0.7, rmin=6, rmax=12,
+                                                     seed=0)).cuda()
+    tau = torch.full(shape, 0.7, device="cuda")
+    k = LBMKernel(shape, tau=0.7, storage="f64", bb="reference", device="cuda")
+    f0 = k.equilibrium()
+    e = torch.as_tensor(k.lat.e, dtype=torch.float64, device="cuda")
+
+    def loss_of(f, Fz):
+        fl = f[..., -1].double()
+        uz = ((fl * e[:, 2].view(-1, 1, 1)).sum(0) + 0.5 * Fz) / fl.sum(0)
+        return torch.where(mask[..., -1] != 0, torch.zeros_like(uz), uz).mean()
+
+    Fg = torch.tensor([0.0, 0.0, F0], dtype=torch.float64, device="cuda", requires_grad=True)
+    tg = tau.clone().requires_grad_(True)
+    fN, _, _, _ = rollout(k, f0, nsteps, force=Fg, tau_field=tg, mask=mask)
+    loss_of(fN, Fg[2]).backward()
+    del fN
+    work = k.workspace(nsteps)
+
+    def loss_at(Fz, tf):
+        f, _, _, _ = k.step(f0, nsteps, mask=mask, force=[0.0, 0.0, Fz], tau_field=tf,
+                            moments=False, work=work)
+        return float(loss_of(f, Fz))
+
+    h = 1e-4 * F0
+    fd = (loss_at(F0 + h, tau) - loss_at(F0 - h, tau)) / (2 * h)
+    assert abs(float(Fg.grad[2]) - fd) / abs(fd) < 1e-6
+    fluid = (mask == 0).nonzero()
+    pick = torch.randperm(len(fluid), generator=torch.Generator().manual_seed(1))[:4]
+    gmax = float(tg.grad.abs().max())
+    for i in pick.tolist():
+        c = tuple(int(v) for v in fluid[i])
+        tp, tm = tau.clone(), tau.clone()
+        tp[c] += 1e-4
+        tm[c] -= 1e-4
+        hs = float(tp[c]) - float(tm[c])
+        fdt = (loss_at(F0, tp) - loss_at(F0, tm)) / hs
+        err = abs(float(tg.grad[c]) - fdt) / max(abs(fdt), 1e-3 * gmax)
+        assert err < 1e-4, (c, float(tg.grad[c]), fdt)
+    del work, k, f0, tg, Fg
+    torch.cuda.empty_cache()
+
+
+def test_config5_full_batch_equals_one_call_per_lattice():
+    from maddening_amd.batch import LatticeBatch
+    specs = workloads.batch_specs(0)
+    rng = np.random.default_rng(0)
+    for s in specs:
+        if s["init"] == "cavity":
+            s["bb"] = "halfway"
+            s["mask"] = workloads.cavity_mask(*s["shape"])
+            s["lid_velocity"] = (0.1, 0.0, 0.0)
+    batch = LatticeBatch(specs, device="cuda")
+    f0s = []
+    for k, s in zip(batch.kernels, specs):
+        if s["init"] == "cavity":
+            f0s.append(k.equilibrium())
+        elif k.D == 2:
+            r, u = workloads.taylor_green_2d(*k.grid_shape, float(rng.uniform(0.01, 0.05)))
+            f0s.append(k.equilibrium(torch.from_numpy(r).float(), torch.from_numpy(u).float()))
+        else:
+            r, u = workloads.taylor_green_3d(k.grid_shape[0], float(rng.uniform(0.01, 0.05)))
+            f0s.append(k.equilibrium(torch.from_numpy(r).float(), torch.from_numpy(u).float()))
+    nsteps = 200
+    batch.set_states(f0s)
+    h = batch.graph(nsteps)
+    batch.launch_graph(h)
+    torch.cuda.synchronize()
+    got = batch.states(nsteps)
+    batch.set_states(f0s)
+    ref = batch.run_per_lattice(nsteps)
+    torch.cuda.synchronize()
+    for i, (a, b) in enumerate(zip(got, ref)):
+        assert torch.equal(a, b), (i, specs[i]["lattice"], specs[i]["shape"])
+    batch.close()
+    del batch, got, ref, f0s
+    torch.cuda.empty_cache()
