@@ -405,3 +405,35 @@ def test_flat_tiles_match_row_blocks_bitwise(lattice, shape, bb, storage):
     torch.cuda.synchronize()
     for a, b in zip(*outs):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("shape,bx", [((65, 64, 256), 0), ((25, 17, 256), 0), ((9, 11, 161), 0),
+                                      ((9, 11, 161), -1), ((40, 32, 64), 64)])
+def test_xcd_run_orders_match_bitwise(shape, bx):
+    """The XCD remap only permutes which workgroup takes which tile: identity,
+    runs of 3 or 512 (with a remainder past the last whole set) and one
+    eighth per XCD give the same bits as the library's own choice, for row
+    blocks and flattened tiles, forward and through the adjoint."""
+    from maddening_amd.adjoint import rollout
+    rng = np.random.default_rng(5)
+    D = len(shape)
+    e, w, _ = orc.lattice(19)
+    f0 = orc.equilibrium(1 + 0.01 * rng.standard_normal(shape),
+                         0.03 * rng.standard_normal(shape + (D,)), e, w)
+    mask = torch.from_numpy((rng.random(shape) < 0.15).astype(np.uint8)).cuda()
+    tau = torch.from_numpy(rng.uniform(0.6, 1.1, shape)).float().cuda()
+    outs = []
+    for remap, G in ((False, 0), (True, 1), (True, 3), (True, 512), (True, 0)):
+        k = kernel(shape, "D3Q19", 0.8, storage="f32", bb="reference", block_x=bx,
+                   xcd_remap=remap, xcd_group=G)
+        f = k.from_aos(torch.from_numpy(f0).cuda())
+        fo, rho, u, _ = k.step(f, 4, mask=mask, tau_field=tau, force=[1e-5, 0.0, 2e-5])
+        tg = tau.clone().requires_grad_(True)
+        fg = f.clone().requires_grad_(True)
+        fN, _, _, _ = rollout(k, fg, 4, tau_field=tg, mask=mask)
+        fN.double().pow(2).sum().backward()
+        outs.append((fo.view(torch.uint8), rho, u, fg.grad, tg.grad))
+    torch.cuda.synchronize()
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert torch.equal(a, b)
