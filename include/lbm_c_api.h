@@ -125,7 +125,7 @@ int lbm_version(void);
 const char* lbm_last_error(void);
 
 /* Bytes of device workspace lbm_step() needs for `nsteps` steps (0 if none):
- * one state for the pull sequence's ping-pong, plus 4 bytes per cell of
+ * one state for the pull sequence's ping-pong, plus 8 bytes per cell of
  * bounce-back link words under LBM_BB_HALFWAY. */
 size_t lbm_workspace_size(const lbm_desc* d, int nsteps);
 
@@ -222,7 +222,7 @@ typedef struct lbm_batch_item {
   int64_t pitch;          /* cells from the start of one population plane of f[0]/f[1] to the
                              next: 0 = the cell count (the lbm_step layout); else at least the
                              cell count. A multiple of 64 keeps every store of a step aligned */
-  uint32_t* links;        /* LBM_BB_HALFWAY only, nullable: 4 bytes per cell of device memory
+  uint32_t* links;        /* LBM_BB_HALFWAY only, nullable: 8 bytes per cell of device memory
                              for link words, rebuilt from the mask at the start of every run;
                              the steps then read one word per cell instead of the neighbours'
                              mask bytes (the same results) */

@@ -13,9 +13,10 @@ a lattice whose cell count is not a multiple of 64 otherwise stores
 misaligned runs and its step moves about a third fewer bytes per second.
 ``state_views`` gives the padded buffers as (Q, *G) views; ``states`` and
 ``set_states`` use the ordinary lbm_step layout. Lattices under halfway
-bounce-back also get a buffer of per-cell link words (4 bytes per cell),
+bounce-back also get a buffer of per-cell link words,
 rebuilt from the mask at the start of every run, so that their steps read
-one word per cell instead of the neighbours' mask bytes.
+one word per cell instead of the neighbours' mask bytes (8 bytes per cell:
+the link word, and the moving-wall links read only next to a moving wall).
 """
 from __future__ import annotations
 
@@ -61,7 +62,7 @@ class LatticeBatch:
             self.buffers.append(tuple(
                 torch.empty(k.Q * pitch, dtype=torch.float32, device=self.device)
                 .view(k.Q, pitch)[:, :k.ncells].view(k.state_shape) for _ in range(2)))
-            self.links.append(torch.empty(k.ncells, dtype=torch.int32, device=self.device)
+            self.links.append(torch.empty(2 * k.ncells, dtype=torch.int32, device=self.device)
                               if (bb == "halfway" and link_words) else None)
             m = s.get("mask")
             self.masks.append(None if m is None else

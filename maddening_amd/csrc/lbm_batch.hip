@@ -82,6 +82,7 @@ __global__ void __launch_bounds__(256)
     a.dst = e.buf[parity ^ 1];
     a.mask = e.mask;
     a.links = e.links;
+    a.links_moving = e.links ? e.links + e.plane : nullptr;
     a.plane = e.pitch;  // cell_update uses plane as the stride between planes
     a.na = e.na; a.nb = e.nb; a.nc = e.nc;
     a.inv_tau = e.inv_tau;
@@ -157,11 +158,13 @@ inline int batch_launch(hipStream_t s, const lbm_batch_plan_t* p, int phase, int
     if (gr.q == 19) {
       if (gr.bb == BB_NONE) rc = group_launch<Lattice<19>, BB_NONE>(s, p->dplan, gr, phase, parity, grid);
       else if (gr.bb == BB_REF) rc = group_launch<Lattice<19>, BB_REF>(s, p->dplan, gr, phase, parity, grid);
-      else rc = group_launch<Lattice<19>, BB_HALF>(s, p->dplan, gr, phase, parity, grid);
+      else if (gr.bb == BB_HALF) rc = group_launch<Lattice<19>, BB_HALF>(s, p->dplan, gr, phase, parity, grid);
+      else rc = group_launch<Lattice<19>, BB_HALF_LINKS>(s, p->dplan, gr, phase, parity, grid);
     } else {
       if (gr.bb == BB_NONE) rc = group_launch<Lattice<9>, BB_NONE>(s, p->dplan, gr, phase, parity, grid);
       else if (gr.bb == BB_REF) rc = group_launch<Lattice<9>, BB_REF>(s, p->dplan, gr, phase, parity, grid);
-      else rc = group_launch<Lattice<9>, BB_HALF>(s, p->dplan, gr, phase, parity, grid);
+      else if (gr.bb == BB_HALF) rc = group_launch<Lattice<9>, BB_HALF>(s, p->dplan, gr, phase, parity, grid);
+      else rc = group_launch<Lattice<9>, BB_HALF_LINKS>(s, p->dplan, gr, phase, parity, grid);
     }
     if (rc) return rc;
   }
@@ -172,10 +175,10 @@ inline int batch_run(hipStream_t s, const lbm_batch_plan_t* p, int nsteps, int g
   for (const LinkJob& j : p->links) {
     if (j.q == 19)
       hipLaunchKernelGGL((k_links<Lattice<19>>), dim3(j.g.nblocks), dim3(j.g.bx), 0, s, j.mask,
-                         j.links, j.g.na, j.g.nb, j.g.nc, j.g.bpr);
+                         j.links, j.links + j.g.plane, j.g.na, j.g.nb, j.g.nc, j.g.bpr);
     else
       hipLaunchKernelGGL((k_links<Lattice<9>>), dim3(j.g.nblocks), dim3(j.g.bx), 0, s, j.mask,
-                         j.links, j.g.na, j.g.nb, j.g.nc, j.g.bpr);
+                         j.links, j.links + j.g.plane, j.g.na, j.g.nb, j.g.nc, j.g.bpr);
     if (int rc = check_launch("k_links launch")) return rc;
   }
   int rc = batch_launch(s, p, 0, 0, grid);
@@ -232,11 +235,16 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     if (it.pitch > 0x7fffffffLL)
       return fail(LBM_ERR_INVALID, "batch item " + std::to_string(i) + ": pitch must fit int32");
   }
+  // the wall-rule kernel of an item: halfway items with link words form
+  // their own class (BB_HALF_LINKS)
+  auto kernel_bb = [&](int i) {
+    return (items[i].bb_mode == LBM_BB_HALFWAY && items[i].links) ? int(BB_HALF_LINKS)
+                                                                   : items[i].bb_mode;
+  };
   std::vector<int> order(nitems);
   for (int i = 0; i < nitems; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    const int kx = items[x].q * 4 + items[x].bb_mode, ky = items[y].q * 4 + items[y].bb_mode;
-    return kx < ky;
+    return items[x].q * 4 + kernel_bb(x) < items[y].q * 4 + kernel_bb(y);
   });
   std::vector<char> host(lbm_batch_plan_size(nitems), 0);
   BatchEntry* out = reinterpret_cast<BatchEntry*>(host.data() + kHeaderBytes);
@@ -249,7 +257,7 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     const Geom& g = geo[i];
     BatchEntry& e = out[j];
     e.q = it.q;
-    e.bb = it.bb_mode;
+    e.bb = kernel_bb(i);
     e.na = g.na; e.nb = g.nb; e.nc = g.nc;
     e.bpr = g.bpr;
     e.plane = g.plane;
@@ -276,8 +284,8 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     e.u = it.u;
     e.p = it.p;
     const long long nt = (g.plane + tile_cells - 1) / tile_cells;
-    if (hp->groups.empty() || hp->groups.back().q != it.q || hp->groups.back().bb != it.bb_mode)
-      hp->groups.push_back(BatchGroup{it.q, it.bb_mode, j, j, tiles, tiles});
+    if (hp->groups.empty() || hp->groups.back().q != it.q || hp->groups.back().bb != e.bb)
+      hp->groups.push_back(BatchGroup{it.q, e.bb, j, j, tiles, tiles});
     hp->groups.back().e1 = j + 1;
     hp->groups.back().t1 = tiles + nt;
     tiles += nt;

@@ -14,10 +14,12 @@ struct FwdArgs {
   const void* src;
   void* dst;
   const uint8_t* mask;
-  // per-cell link words for halfway bounce-back (k_links), or null: bit q
-  // set when the cell is fluid and x - e_q is a wall, kLinkMoving when one of
-  // those walls moves, kLinkSolid for a wall cell
+  // per-cell link words for halfway bounce-back (k_links; read by the
+  // BB_HALF_LINKS kernels): bit q set when the cell is fluid and x - e_q is a
+  // wall, kLinkMoving when one of those walls moves, kLinkSolid for a wall
+  // cell; links_moving[cell] has bit q set when x - e_q is a moving wall
   const uint32_t* links;
+  const uint32_t* links_moving;
   const float* force_field;
   const float* tau_field;
   float* rho;
@@ -32,6 +34,9 @@ struct FwdArgs {
   // flattened lattice instead of row blocks (flat_remap: their XCD runs)
   int flat_blocks;
   int flat_remap;
+  // flattened tiles of 32- or 16-bit storage whose plane size is not a
+  // multiple of 64 cells: stores go through LDS (k_cell_staged)
+  int flat_staged;
   C inv_tau;
   C Fa, Fb, Fc;
   C lid_a, lid_b, lid_c;
@@ -239,11 +244,11 @@ __device__ __forceinline__ void gather_state(const T* src, const uint8_t* mask, 
 // Halfway bounce-back gather from a link word (see FwdArgs::links): the same
 // values as gather_state<L, T, BB_HALF, true>, with one aligned 4-byte read
 // in place of the cell's own and its 18 neighbours' mask bytes. A wave with
-// no blocked link skips the wall code entirely; only a link to a moving wall
-// reads that wall's mask byte.
+// no blocked link skips the wall code entirely; a cell next to a moving wall
+// also reads its moving-link word, and no neighbour index is needed.
 template <class L, class T>
-__device__ __forceinline__ void gather_links(const T* src, const uint8_t* mask, long long P,
-                                             const Nbr& n, uint32_t lk,
+__device__ __forceinline__ void gather_links(const T* src, const uint32_t* links_moving,
+                                             long long P, const Nbr& n, uint32_t lk,
                                              typename Storage<T>::C lid_a,
                                              typename Storage<T>::C lid_b,
                                              typename Storage<T>::C lid_c,
@@ -258,42 +263,44 @@ __device__ __forceinline__ void gather_links(const T* src, const uint8_t* mask, 
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
   if (lk) {
+    const uint32_t mv = (lk & kLinkMoving) ? links_moving[n.cell] : 0u;
 #pragma unroll
     for (int q = 1; q < L::Q; ++q) {
       if ((lk >> q) & 1u) {
-        C lid = C(0);
-        if ((lk & kLinkMoving) && mask[n.template at<L>(q, -1)] == 2)
-          lid = C(6) * weight<L, C>(q) *
-                (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c);
+        // 2 w_q (e_q . u_wall) / cs2 where the wall moves (density 1)
+        const C lid = ((mv >> q) & 1u) ? C(6) * weight<L, C>(q) *
+                                             (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b +
+                                              C(L::ec[q]) * lid_c)
+                                       : C(0);
         d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) + lid;
       }
     }
   }
 }
 
+// The cell's update up to the store: d receives the state to write (after
+// the collision when COLLIDE); returns the cell's index.
 // PULL: gather (stream + bounce-back) instead of an aligned read.
 // COLLIDE: apply the collision after the read.
-// STORE: write the state back (false for the moments-only pass).
-template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
-__device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C>& a, int row,
-                                            int ic) {
+template <class L, class T, int BB, bool PULL, bool COLLIDE>
+__device__ __forceinline__ int cell_compute(const FwdArgs<typename Storage<T>::C>& a, int row,
+                                            int ic, typename Storage<T>::C (&d)[L::Q]) {
   using C = typename Storage<T>::C;
-  using St = Storage<T>;
   const int ia = row / a.nb;
   const int ib = row - ia * a.nb;
   Nbr n;
   n.init(ia, ib, ic, a.na, a.nb, a.nc);
   const T* src = static_cast<const T*>(a.src);
   const long long P = a.plane;
-  C d[L::Q];
   uint8_t self;
-  if (BB == BB_HALF && PULL && a.links) {
+  if constexpr (BB == BB_HALF_LINKS && PULL) {
     const uint32_t lk = a.links[n.cell];
     self = (lk & kLinkSolid) ? uint8_t(1) : uint8_t(0);
-    gather_links<L, T>(src, a.mask, P, n, lk, a.lid_a, a.lid_b, a.lid_c, d);
+    gather_links<L, T>(src, a.links_moving, P, n, lk, a.lid_a, a.lid_b, a.lid_c, d);
   } else {
     self = a.mask ? a.mask[n.cell] : uint8_t(0);
-    gather_state<L, T, BB, PULL>(src, a.mask, P, n, self, a.lid_a, a.lid_b, a.lid_c, d);
+    gather_state<L, T, BB == BB_HALF_LINKS ? BB_HALF : BB, PULL>(src, a.mask, P, n, self, a.lid_a,
+                                                                 a.lid_b, a.lid_c, d);
   }
   if (PULL && a.zh_n && !self) apply_faces<L, C>(a, ia, ib, ic, d);
 
@@ -306,40 +313,101 @@ __device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C
     const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
     collide_for<L, T, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
   }
+  return n.cell;
+}
+
+// cell_compute, then the store (STORE false: the moments-only pass).
+template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
+__device__ __forceinline__ void cell_update(const FwdArgs<typename Storage<T>::C>& a, int row,
+                                            int ic) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  C d[L::Q];
+  const int cell = cell_compute<L, T, BB, PULL, COLLIDE>(a, row, ic, d);
   if (STORE) {
     T* dst = static_cast<T*>(a.dst);
 #pragma unroll
-    for (int q = 0; q < L::Q; ++q) St::template store<L>(dst + q * P + n.cell, q, d[q]);
+    for (int q = 0; q < L::Q; ++q) St::template store<L>(dst + q * a.plane + cell, q, d[q]);
   }
 }
 
 template <class L, class T, int BB, bool PULL, bool COLLIDE, bool STORE>
 __global__ void __launch_bounds__(256)
     k_cell(FwdArgs<typename Storage<T>::C> a) {
+  int row, ic;
   if (a.flat_blocks > 0) {
     // 256-thread tiles of the flattened lattice: every store of a wave is
     // one aligned 256-byte run whenever the plane size is a multiple of 64
     const int lb = xcd_remap(blockIdx.x, a.flat_blocks, a.flat_remap);
     const int cell = lb * 256 + (int)threadIdx.x;
     if (cell >= (int)a.plane) return;
-    const int row = (int)((unsigned)cell / (unsigned)a.nc);
-    cell_update<L, T, BB, PULL, COLLIDE, STORE>(a, row, cell - row * a.nc);
-    return;
+    row = (int)((unsigned)cell / (unsigned)a.nc);
+    ic = cell - row * a.nc;
+  } else {
+    const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
+    row = lb / a.bpr;
+    ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
+    if (ic >= a.nc) return;
   }
-  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
-  const int row = lb / a.bpr;
-  const int ic = (lb - row * a.bpr) * blockDim.x + threadIdx.x;
-  if (ic >= a.nc) return;
+  // one inlined update for both launch shapes keeps the register count of
+  // a single copy
   cell_update<L, T, BB, PULL, COLLIDE, STORE>(a, row, ic);
 }
 
-// Link words for halfway bounce-back (FwdArgs::links), built at the start of
-// each lbm_step call or batch run (the mask is an input of each): reads the
-// cell's mask byte and its neighbours' (from cache), writes 4 bytes per cell.
+// Flattened 256-cell tiles of a lattice whose plane size is not a multiple
+// of 64 cells. Storing straight from registers, every wave's 64-element run
+// would start part-way into a 64-element segment of each population plane
+// (plane q starts at q * plane). The tile's results go through LDS instead,
+// and each wave stores one 64-element run aligned in the destination plane;
+// only the two runs at the tile's ends stay partial (they share segments
+// with the neighbouring tiles). 32- and 16-bit storage only (19.5 KB of LDS
+// for D3Q19 float).
+template <class L, class T, int BB, bool PULL, bool COLLIDE>
+__global__ void __launch_bounds__(256)
+    k_cell_staged(FwdArgs<typename Storage<T>::C> a) {
+  using C = typename Storage<T>::C;
+  using St = Storage<T>;
+  __shared__ T stage[L::Q][256];
+  const int lb = xcd_remap(blockIdx.x, a.flat_blocks, a.flat_remap);
+  const int t0 = lb * 256;
+  const int plane = (int)a.plane;
+  const int n = plane - t0 < 256 ? plane - t0 : 256;  // cells in this tile
+  const int tid = (int)threadIdx.x;
+  if (tid < n) {
+    const int cell = t0 + tid;
+    const int row = (int)((unsigned)cell / (unsigned)a.nc);
+    C d[L::Q];
+    cell_compute<L, T, BB, PULL, COLLIDE>(a, row, cell - row * a.nc, d);
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) stage[q][tid] = St::template enc<L>(q, d[q]);
+  }
+  __syncthreads();
+  T* dst = static_cast<T*>(a.dst);
+  const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    // element t of the tile goes to dst[base + t]; the first element at a
+    // multiple of 64 is t = lead. Wave w stores t in [lead + 64 (w - 1),
+    // lead + 64 w), wave 0 also [lead + 192, 256): every t in [0, n) once.
+    const long long base = (long long)q * plane + t0;
+    const int lead = (int)((64 - (base & 63)) & 63);
+    int t = lead - 64 + 64 * wave + lane;
+    if (t >= 0 && t < n) St::put(dst + base + t, stage[q][t]);
+    if (wave == 0) {
+      t = lead + 192 + lane;
+      if (t < n) St::put(dst + base + t, stage[q][t]);
+    }
+  }
+}
+
+// Link words for halfway bounce-back (FwdArgs::links, links_moving), built
+// at the start of each lbm_step call or batch run (the mask is an input of
+// each): reads the cell's mask byte and its neighbours' (from cache), writes
+// 8 bytes per cell.
 template <class L>
 __global__ void __launch_bounds__(256)
-    k_links(const uint8_t* __restrict__ mask, uint32_t* __restrict__ links, int na, int nb, int nc,
-            int bpr) {
+    k_links(const uint8_t* __restrict__ mask, uint32_t* __restrict__ links,
+            uint32_t* __restrict__ links_moving, int na, int nb, int nc, int bpr) {
   const int row = blockIdx.x / bpr;
   const int ic = (blockIdx.x - row * bpr) * blockDim.x + threadIdx.x;
   if (ic >= nc) return;
@@ -347,17 +415,20 @@ __global__ void __launch_bounds__(256)
   const int ib = row - ia * nb;
   Nbr n;
   n.init(ia, ib, ic, na, nb, nc);
-  uint32_t w = 0;
+  uint32_t w = 0, mv = 0;
   if (mask[n.cell]) {
     w = kLinkSolid;
   } else {
 #pragma unroll
     for (int q = 1; q < L::Q; ++q) {
       const uint8_t m = mask[n.template at<L>(q, -1)];
-      if (m) w |= (1u << q) | (m == 2 ? kLinkMoving : 0u);
+      if (m) w |= 1u << q;
+      if (m == 2) mv |= 1u << q;
     }
+    if (mv) w |= kLinkMoving;
   }
   links[n.cell] = w;
+  links_moving[n.cell] = mv;
 }
 
 }  // namespace mlbm

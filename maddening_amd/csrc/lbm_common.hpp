@@ -51,6 +51,8 @@ __device__ __forceinline__ constexpr C weight(int q) {
 // ---------------------------------------------------------------------------
 template <class T> struct Storage;
 
+// Each trait: load (stored value -> deviation), enc (deviation -> stored
+// value), put (non-temporal store of a stored value) and store = put(enc).
 template <> struct Storage<float> {
   using C = float;
   template <class L>
@@ -58,9 +60,10 @@ template <> struct Storage<float> {
     return *p - weight<L, float>(q);
   }
   template <class L>
-  static __device__ __forceinline__ void store(float* p, int q, float d) {
-    __builtin_nontemporal_store(weight<L, float>(q) + d, p);
-  }
+  static __device__ __forceinline__ float enc(int q, float d) { return weight<L, float>(q) + d; }
+  static __device__ __forceinline__ void put(float* p, float v) { __builtin_nontemporal_store(v, p); }
+  template <class L>
+  static __device__ __forceinline__ void store(float* p, int q, float d) { put(p, enc<L>(q, d)); }
 };
 
 template <> struct Storage<double> {
@@ -70,9 +73,10 @@ template <> struct Storage<double> {
     return *p - weight<L, double>(q);
   }
   template <class L>
-  static __device__ __forceinline__ void store(double* p, int q, double d) {
-    __builtin_nontemporal_store(weight<L, double>(q) + d, p);
-  }
+  static __device__ __forceinline__ double enc(int q, double d) { return weight<L, double>(q) + d; }
+  static __device__ __forceinline__ void put(double* p, double v) { __builtin_nontemporal_store(v, p); }
+  template <class L>
+  static __device__ __forceinline__ void store(double* p, int q, double d) { put(p, enc<L>(q, d)); }
 };
 
 // float32 deviation f - w: same 4 bytes as f32, but the rounding of the
@@ -88,9 +92,12 @@ template <> struct Storage<f32dev_t> {
     return p->v;
   }
   template <class L>
-  static __device__ __forceinline__ void store(f32dev_t* p, int, float d) {
-    __builtin_nontemporal_store(d, &p->v);
+  static __device__ __forceinline__ f32dev_t enc(int, float d) { return f32dev_t{d}; }
+  static __device__ __forceinline__ void put(f32dev_t* p, f32dev_t v) {
+    __builtin_nontemporal_store(v.v, &p->v);
   }
+  template <class L>
+  static __device__ __forceinline__ void store(f32dev_t* p, int q, float d) { put(p, enc<L>(q, d)); }
 };
 
 // IEEE half holds (f - w) * 2^10: deviations stay below ~1, so the scaled
@@ -105,9 +112,12 @@ template <> struct Storage<_Float16> {
     return (float)(*p) * (1.0f / kHalfScale);
   }
   template <class L>
-  static __device__ __forceinline__ void store(_Float16* p, int, float d) {
-    __builtin_nontemporal_store((_Float16)(d * kHalfScale), p);
+  static __device__ __forceinline__ _Float16 enc(int, float d) { return (_Float16)(d * kHalfScale); }
+  static __device__ __forceinline__ void put(_Float16* p, _Float16 v) {
+    __builtin_nontemporal_store(v, p);
   }
+  template <class L>
+  static __device__ __forceinline__ void store(_Float16* p, int q, float d) { put(p, enc<L>(q, d)); }
 };
 
 template <> struct Storage<__bf16> {
@@ -117,10 +127,10 @@ template <> struct Storage<__bf16> {
     return (float)(*p);
   }
   template <class L>
-  static __device__ __forceinline__ void store(__bf16* p, int, float d) {
-    __bf16 v = (__bf16)d;
-    __builtin_nontemporal_store(v, p);
-  }
+  static __device__ __forceinline__ __bf16 enc(int, float d) { return (__bf16)d; }
+  static __device__ __forceinline__ void put(__bf16* p, __bf16 v) { __builtin_nontemporal_store(v, p); }
+  template <class L>
+  static __device__ __forceinline__ void store(__bf16* p, int q, float d) { put(p, enc<L>(q, d)); }
 };
 
 // ---------------------------------------------------------------------------

@@ -401,6 +401,7 @@ FwdArgs<C> make_args(const lbm_desc* d, const Geom& g) {
   a.nblocks = g.nblocks;
   a.remap = xcd_group(d, g);
   a.flat_blocks = flat_tiles(d, g);
+  a.flat_staged = (a.flat_blocks > 0 && g.plane % 64 != 0 && storage_size(d->storage) <= 4) ? 1 : 0;
   if (a.flat_blocks && (d->flags & LBM_FLAG_XCD_REMAP))
     a.flat_remap = d->xcd_group > 0 ? d->xcd_group : a.flat_blocks / 8;
   a.inv_tau = C(1.0 / d->tau);
@@ -461,6 +462,13 @@ int launch_cell(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>:
     }
     if (g.rowv == 2) return launch_row<L, T, 2, PULL, COLLIDE>(s, g, a);
     if (g.rowv == 1) return launch_row<L, T, 1, PULL, COLLIDE>(s, g, a);
+  }
+  if constexpr (STORE && sizeof(T) <= 4) {
+    if (a.flat_staged) {
+      hipLaunchKernelGGL((k_cell_staged<L, T, BB, PULL, COLLIDE>), dim3(a.flat_blocks), dim3(256),
+                         0, s, a);
+      return check_launch("k_cell_staged launch");
+    }
   }
   if (a.flat_blocks > 0)
     hipLaunchKernelGGL((k_cell<L, T, BB, PULL, COLLIDE, STORE>), dim3(a.flat_blocks), dim3(256), 0,
@@ -553,11 +561,12 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
       case LBM_BB_REFERENCE: return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
       default: {
         uint32_t* links = reinterpret_cast<uint32_t*>(static_cast<char*>(work) + links_offset(d, g));
-        hipLaunchKernelGGL((k_links<L>), dim3(g.nblocks), dim3(g.bx), 0, s, mask, links, g.na,
-                           g.nb, g.nc, g.bpr);
+        hipLaunchKernelGGL((k_links<L>), dim3(g.nblocks), dim3(g.bx), 0, s, mask, links,
+                           links + g.plane, g.na, g.nb, g.nc, g.bpr);
         if (int rc = check_launch("k_links launch")) return rc;
         a.links = links;
-        return step_pull<L, T, BB_HALF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+        a.links_moving = links + g.plane;
+        return step_pull<L, T, BB_HALF_LINKS>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
       }
     }
   }
@@ -722,7 +731,7 @@ size_t lbm_workspace_size(const lbm_desc* d, int nsteps) {
   const int scheme = resolve_scheme(d, nsteps);
   if (scheme == LBM_SCHEME_PUSH && nsteps <= 1) return 0;
   if (scheme == LBM_SCHEME_PULL && d->bb_mode == LBM_BB_HALFWAY)
-    return links_offset(d, g) + (size_t)g.plane * sizeof(uint32_t);
+    return links_offset(d, g) + 2 * (size_t)g.plane * sizeof(uint32_t);
   return (size_t)g.plane * (size_t)d->q * (size_t)storage_size(d->storage);
 }
 

@@ -10,7 +10,10 @@
 
 namespace mlbm {
 
-enum { BB_NONE = 0, BB_REF = 1, BB_HALF = 2 };
+// BB_HALF_LINKS: halfway bounce-back whose pull gathers read per-cell link
+// words (FwdArgs::links) instead of the neighbours' mask bytes; a separate
+// instantiation, so neither gather carries the other's registers.
+enum { BB_NONE = 0, BB_REF = 1, BB_HALF = 2, BB_HALF_LINKS = 3 };
 
 // message for lbm_last_error(), one per host thread
 inline thread_local std::string g_last_error;

@@ -71,12 +71,12 @@ def test_host_validation_errors_without_gpu():
     d.bb_mode, d.flags = 0, 0
     assert L.lbm_workspace_size(ctypes.byref(d), 10) == 8 ** 3 * 19 * 4
     assert L.lbm_state_bytes(ctypes.byref(d)) == 8 ** 3 * 19 * 4
-    # halfway bounce-back adds 4-byte link words per cell after the state,
-    # at a 256-byte aligned offset
+    # halfway bounce-back adds two 4-byte link words per cell after the
+    # state, at a 256-byte aligned offset
     d.bb_mode = 2
-    assert L.lbm_workspace_size(ctypes.byref(d), 10) == 8 ** 3 * 19 * 4 + 8 ** 3 * 4
+    assert L.lbm_workspace_size(ctypes.byref(d), 10) == 8 ** 3 * 19 * 4 + 8 ** 3 * 8
     d.storage, d.dims[0], d.dims[1], d.dims[2] = 2, 3, 5, 7
-    assert L.lbm_workspace_size(ctypes.byref(d), 10) == 4096 + 105 * 4
+    assert L.lbm_workspace_size(ctypes.byref(d), 10) == 4096 + 105 * 8
 
 
 def test_batch_item_validation_without_gpu():
