@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LBM_API_VERSION 5
+#define LBM_API_VERSION 6
 
 typedef enum lbm_status {
   LBM_OK = 0,
@@ -161,10 +161,24 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out,
  *            from g_{n-1} by streaming and bounce-back;
  *   phase 2: src = post-collision g_{n-1}, dst = f_n, and the moments of f_n
  *            into rho/u/p (each nullable).
+ * links (nullable, LBM_BB_HALFWAY only): the link words lbm_links_build made
+ * from this mask; the gathers then read one word per cell instead of the
+ * neighbours' mask bytes (the same results).
  */
 int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, void* dst,
-                   const uint8_t* mask, const float* force_field, const float* tau_field,
-                   float* rho, float* u, float* p);
+                   const uint8_t* mask, const uint32_t* links, const float* force_field,
+                   const float* tau_field, float* rho, float* u, float* p);
+
+/*
+ * Link words of a wall mask for LBM_BB_HALFWAY (no counterpart in the
+ * reference: they restate its mask, lbm.py:914-919, per link). Per cell, two
+ * 32-bit words: which directions q have a wall at x - e_q, whether one of
+ * those walls moves (mask 2) and whether the cell is a wall; then which links
+ * end at a moving wall. lbm_links_size bytes of device memory; rebuild after
+ * the mask changes. lbm_step builds its own inside its workspace.
+ */
+size_t lbm_links_size(const lbm_desc* d);
+int lbm_links_build(void* stream, const lbm_desc* d, const uint8_t* mask, uint32_t* links);
 
 /*
  * Output moments of a state without stepping: _compute_macroscopic with the
@@ -285,12 +299,12 @@ int lbm_adjoint_begin(void* stream, const lbm_desc* d, void* work);
  * and 1 write) and f_n is gathered from it by the forward's own streaming
  * and bounce-back. Adds dL/dtau(x) into tau_bar (*G) and dL/dF(x) into
  * force_bar_field (*G, D) when those are non-NULL; adds uniform-force
- * partials into work when work is non-NULL. The other arguments mean what
- * they mean in lbm_step.
+ * partials into work when work is non-NULL. links: as in lbm_pull_phase.
+ * The other arguments mean what they mean in lbm_step.
  */
 int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, int f_n_post,
                      const void* fbar_next, void* fbar_out, const uint8_t* mask,
-                     const float* force_field, const float* tau_field,
+                     const uint32_t* links, const float* force_field, const float* tau_field,
                      float* tau_bar, float* force_bar_field, void* work);
 
 /*

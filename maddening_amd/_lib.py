@@ -116,9 +116,11 @@ _PROTOS = {
     "lbm_adjoint_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(LbmDesc)]),
     "lbm_adjoint_begin": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp]),
     "lbm_adjoint_step": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, ctypes.c_int, _vp,
-                                        _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+                                        _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lbm_pull_phase": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), ctypes.c_int, _vp, _vp, _vp,
-                                      _vp, _vp, _vp, _vp, _vp]),
+                                      _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lbm_links_size": (ctypes.c_size_t, [ctypes.POINTER(LbmDesc)]),
+    "lbm_links_build": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),
     "lbm_moments_adjoint": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp, _vp, _vp,
                                            _vp, _vp, _vp, _vp, _vp, _vp]),
     "lbm_adjoint_end": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),

@@ -37,10 +37,21 @@ def segment_length(nsteps: int, checkpoint: Union[str, int]) -> int:
     return k
 
 
-def _pull(L, st, d, phase, src, dst, m, ff, tf, rho=None, u=None, p=None):
+def _pull(L, st, d, phase, src, dst, m, links, ff, tf, rho=None, u=None, p=None):
     _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), phase, _ptr(src), _ptr(dst), _ptr(m),
-                                _ptr(ff), _ptr(tf), _ptr(rho), _ptr(u), _ptr(p)),
+                                _ptr(links), _ptr(ff), _ptr(tf), _ptr(rho), _ptr(u), _ptr(p)),
                "lbm_pull_phase")
+
+
+def _links(L, st, d, kern, m):
+    """Halfway link words of the mask (lbm_links_build), read by every
+    launch of the rollout in place of the neighbours' mask bytes."""
+    if kern.bb != "halfway":
+        return None
+    links = torch.empty(L.lbm_links_size(ctypes.byref(d)) // 4, dtype=torch.int32,
+                        device=kern.device)
+    _lib.check(L.lbm_links_build(st, ctypes.byref(d), _ptr(m), _ptr(links)), "lbm_links_build")
+    return links
 
 
 class _Rollout(torch.autograd.Function):
@@ -65,6 +76,7 @@ class _Rollout(torch.autograd.Function):
         if kern.bb != "none" and m is None:
             raise ValueError(f"bb='{kern.bb}' needs a wall mask")
         tf = None if tau_field is None else tau_field.to(dev, torch.float32).contiguous()
+        links = _links(L, st, d, kern, m)
         ckpts = [f0]           # source of each segment's first state
         # the last segment's states come straight from this pass: no recompute
         s_last = ((nsteps - 1) // seg) * seg
@@ -75,7 +87,7 @@ class _Rollout(torch.autograd.Function):
             ckpt = (k + 1) % seg == 0 and k + 1 < nsteps   # g_k starts segment k + 1
             in_tail = s_last <= k <= nsteps - 2
             g = kern.empty() if (ckpt or in_tail) else pool[k & 1]
-            _pull(L, st, d, 0 if k == 0 else 1, prev, g, m, ff, tf)
+            _pull(L, st, d, 0 if k == 0 else 1, prev, g, m, links, ff, tf)
             if ckpt:
                 ckpts.append(g)
             if in_tail:
@@ -85,8 +97,9 @@ class _Rollout(torch.autograd.Function):
         rho = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
         u = torch.empty(kern.grid_shape + (kern.D,), dtype=torch.float32, device=dev)
         p = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
-        _pull(L, st, d, 2, prev, fN, m, ff, tf, rho, u, p)
+        _pull(L, st, d, 2, prev, fN, m, links, ff, tf, rho, u, p)
         ctx.kern = kern
+        ctx.links = links
         ctx.nsteps = nsteps
         ctx.seg = seg
         ctx.mask = mask
@@ -165,15 +178,16 @@ class _Rollout(torch.autograd.Function):
                 states += list(tail)
             for i in range(len(states), n):
                 g = kern.empty()
-                _pull(L, st, d, 1 if (s > 0 or i > 1) else 0, states[-1], g, m, ff, tf)
+                _pull(L, st, d, 1 if (s > 0 or i > 1) else 0, states[-1], g, m, ctx.links, ff, tf)
                 states.append(g)
             for i in reversed(range(n)):
                 post = int(s > 0 or i > 0)
                 dk = d_mid if post else d_first
                 _lib.check(L.lbm_adjoint_step(st, ctypes.byref(dk), _ptr(states[i]), post,
-                                              _ptr(fbar), _ptr(spare), _ptr(m), _ptr(ff),
-                                              _ptr(tf), _ptr(tau_bar), _ptr(fbar_field),
-                                              _ptr(work)), "lbm_adjoint_step")
+                                              _ptr(fbar), _ptr(spare), _ptr(m),
+                                              _ptr(ctx.links), _ptr(ff), _ptr(tf),
+                                              _ptr(tau_bar), _ptr(fbar_field), _ptr(work)),
+                           "lbm_adjoint_step")
                 fbar, spare = spare, fbar
             del states
         grad_force = None

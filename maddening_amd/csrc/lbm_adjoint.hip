@@ -27,6 +27,9 @@ struct AdjArgs {
   const C* gin;         // dL/df_{n+1}
   C* gout;              // dL/df_n
   const uint8_t* mask;
+  // halfway link words (lbm_links_build), read by the BB_HALF_LINKS kernels
+  const uint32_t* links;
+  const uint32_t* links_moving;
   const float* force_field;
   const float* tau_field;
   float* tau_bar;       // += dL/dtau(x)
@@ -166,12 +169,21 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
     Nbr n;
     n.init(ia, ib, ic, a.na, a.nb, a.nc);
     const long long P = a.plane;
-    const uint8_t self = a.mask ? a.mask[n.cell] : uint8_t(0);
-    if (BB == BB_HALF && self) {
-      // a solid cell is reset to rest every step: nothing flows back into it
-#pragma unroll
-      for (int q = 0; q < L::Q; ++q) __builtin_nontemporal_store(C(0), a.gout + q * P + n.cell);
+    uint32_t lk = 0;
+    uint8_t self;
+    if constexpr (BB == BB_HALF_LINKS) {
+      lk = a.links[n.cell];
+      self = (lk & kLinkSolid) ? uint8_t(1) : uint8_t(0);
     } else {
+      self = a.mask ? a.mask[n.cell] : uint8_t(0);
+    }
+    // Under halfway bounce-back a solid cell is reset to rest every step:
+    // nothing flows back into it, and its tau and force do not act. Its lanes
+    // still run the whole update (on valid addresses) and only the results
+    // are zeroed, so a wave mixing solid and fluid cells issues one load and
+    // one store per slot instead of a branch per kind.
+    const bool solid = (BB == BB_HALF || BB == BB_HALF_LINKS) && self;
+    {
       // (cotangents are written with non-temporal stores, as the forward
       // writes its states: each is read once, by the next transposed step)
       // g_bar = S^T B^T f'_bar: the value the forward step moved out of
@@ -189,6 +201,12 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
           gb[p] = a.gin[p * P + x];
         } else if (BB == BB_REF) {
           gb[p] = a.gin[(a.mask[x] ? L::opp[p] : p) * P + x];
+        } else if (BB == BB_HALF_LINKS) {
+          // x = cell + e_p is a wall exactly when this cell's link opp(p) is
+          // blocked (bit 0 is never set): one load, no neighbour mask
+          const C* ptr = ((lk >> L::opp[p]) & 1u) ? a.gin + L::opp[p] * P + n.cell
+                                                   : a.gin + p * P + x;
+          gb[p] = *ptr;
         } else {
           // streamed slot loaded first (valid address either way), so only
           // blocked links wait for the neighbour's mask
@@ -198,8 +216,12 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       }
       // forward quantities of this cell
       C d[L::Q];
-      gather_state<L, T, BB, FPOST>(static_cast<const T*>(a.f), a.mask, P, n, self, a.lid_a,
-                                    a.lid_b, a.lid_c, d);
+      if constexpr (BB == BB_HALF_LINKS && FPOST)
+        gather_links<L, T>(static_cast<const T*>(a.f), a.links_moving, P, n, lk, a.lid_a, a.lid_b,
+                           a.lid_c, d);
+      else
+        gather_state<L, T, BB == BB_HALF_LINKS ? BB_HALF : BB, FPOST>(
+            static_cast<const T*>(a.f), a.mask, P, n, self, a.lid_a, a.lid_b, a.lid_c, d);
       // the state the step started from includes the previous step's faces
       if (FPOST && FACES && !self) apply_faces<L, C>(a, ia, ib, ic, d);
       C Fa, Fb, Fc;
@@ -273,24 +295,29 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
           o[p] = keep * gb[p] + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
         C rbf[2] = {C(0), C(0)};
         apply_faces_transpose<L, C>(a, ia, ib, ic, o, rbf);
-        rb0 = double(rbf[0]);
-        rb1 = double(rbf[1]);
+        rb0 = solid ? 0.0 : double(rbf[0]);
+        rb1 = solid ? 0.0 : double(rbf[1]);
 #pragma unroll
-        for (int p = 0; p < L::Q; ++p) __builtin_nontemporal_store(o[p], a.gout + p * P + n.cell);
+        for (int p = 0; p < L::Q; ++p)
+          __builtin_nontemporal_store(solid ? C(0) : o[p], a.gout + p * P + n.cell);
       } else {
 #pragma unroll
         for (int p = 0; p < L::Q; ++p) {
           const C eB = C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
           const C g = xs ? gb[L::opp[p]] : gb[p];
-          __builtin_nontemporal_store(keep * g + A + (xs ? -eB : eB), a.gout + p * P + n.cell);
+          __builtin_nontemporal_store(solid ? C(0) : keep * g + A + (xs ? -eB : eB),
+                                      a.gout + p * P + n.cell);
         }
       }
-      if (a.tau_bar) __builtin_nontemporal_store(tau_bar_in + float(-om * om * tb), a.tau_bar + n.cell);
+      if (a.tau_bar && !solid)
+        __builtin_nontemporal_store(tau_bar_in + float(-om * om * tb), a.tau_bar + n.cell);
       // dL/dF: the source directly (u fixed) plus through u = (j + F/2)/rho
-      dFa = c * (C(3) * Ga - C(3) * ua * G0 + C(9) * Ka) + C(0.5) * Ba;
-      dFb = c * (C(3) * Gb - C(3) * ub * G0 + C(9) * Kb) + C(0.5) * Bb;
-      dFc = c * (C(3) * Gc - C(3) * uc * G0 + C(9) * Kc) + C(0.5) * Bc;
-      if (a.force_bar) add_force_bar<L, C>(a, n.cell, dFa, dFb, dFc);
+      if (!solid) {
+        dFa = c * (C(3) * Ga - C(3) * ua * G0 + C(9) * Ka) + C(0.5) * Ba;
+        dFb = c * (C(3) * Gb - C(3) * ub * G0 + C(9) * Kb) + C(0.5) * Bb;
+        dFc = c * (C(3) * Gc - C(3) * uc * G0 + C(9) * Kc) + C(0.5) * Bc;
+        if (a.force_bar) add_force_bar<L, C>(a, n.cell, dFa, dFb, dFc);
+      }
     }
   }
   if (a.partial)
@@ -497,22 +524,30 @@ int adjoint_launch(hipStream_t s, int bb, bool in_swapped, const Geom& g,
                            dim3(256), 0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF, FPOST, false, FACES>), dim3(g.nblocks),
-                         dim3(256), 0, s, a);
+      if (a.links)
+        hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF_LINKS, FPOST, false, FACES>), dim3(g.nblocks),
+                           dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_adjoint<L, T, BB_HALF, FPOST, false, FACES>), dim3(g.nblocks),
+                           dim3(256), 0, s, a);
   }
   return check_launch("k_adjoint launch");
 }
 
 template <class L, class T>
 int adjoint_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f, int f_post,
-                  const void* gin, void* gout, const uint8_t* mask, const float* ff,
-                  const float* tf, float* tau_bar, float* force_bar, void* work) {
+                  const void* gin, void* gout, const uint8_t* mask, const uint32_t* links,
+                  const float* ff, const float* tf, float* tau_bar, float* force_bar, void* work) {
   using C = typename Storage<T>::C;
   AdjArgs<C> a = adj_args<C>(d, g);
   a.f = f;
   a.gin = static_cast<const C*>(gin);
   a.gout = static_cast<C*>(gout);
   a.mask = d->bb_mode == LBM_BB_NONE ? nullptr : mask;
+  if (d->bb_mode == LBM_BB_HALFWAY && links) {
+    a.links = links;
+    a.links_moving = links + g.plane;
+  }
   a.force_field = ff;
   a.tau_field = tf;
   a.tau_bar = tau_bar;
@@ -576,8 +611,8 @@ int lbm_adjoint_begin(void* stream, const lbm_desc* d, void* work) {
 
 int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, int f_n_post,
                      const void* fbar_next, void* fbar_out, const uint8_t* mask,
-                     const float* force_field, const float* tau_field, float* tau_bar,
-                     float* force_bar_field, void* work) {
+                     const uint32_t* links, const float* force_field, const float* tau_field,
+                     float* tau_bar, float* force_bar_field, void* work) {
   g_last_error.clear();
   (void)hipGetLastError();
   Geom g;
@@ -588,24 +623,18 @@ int lbm_adjoint_step(void* stream, const lbm_desc* d, const void* f_n, int f_n_p
   if (fbar_next == fbar_out) return fail(LBM_ERR_INVALID, "fbar_next and fbar_out must be distinct");
   if (d->bb_mode != LBM_BB_NONE && !mask) return fail(LBM_ERR_INVALID, "bb_mode needs a wall mask");
   hipStream_t s = (hipStream_t)stream;
+#define MLBM_ADJ(LAT, T)                                                                    \
+  return adjoint_typed<LAT, T>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, links,    \
+                               force_field, tau_field, tau_bar, force_bar_field, work)
   if (d->q == 19) {
-    if (d->storage == LBM_F32_DEV)
-      return adjoint_typed<Lattice<19>, f32dev_t>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
-                                                 tau_field, tau_bar, force_bar_field, work);
-    if (d->storage == LBM_F32)
-      return adjoint_typed<Lattice<19>, float>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
-                                              tau_field, tau_bar, force_bar_field, work);
-    return adjoint_typed<Lattice<19>, double>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
-                                             tau_field, tau_bar, force_bar_field, work);
+    if (d->storage == LBM_F32_DEV) MLBM_ADJ(Lattice<19>, f32dev_t);
+    if (d->storage == LBM_F32) MLBM_ADJ(Lattice<19>, float);
+    MLBM_ADJ(Lattice<19>, double);
   }
-  if (d->storage == LBM_F32_DEV)
-    return adjoint_typed<Lattice<9>, f32dev_t>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
-                                              tau_field, tau_bar, force_bar_field, work);
-  if (d->storage == LBM_F32)
-    return adjoint_typed<Lattice<9>, float>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
-                                           tau_field, tau_bar, force_bar_field, work);
-  return adjoint_typed<Lattice<9>, double>(s, d, g, f_n, f_n_post, fbar_next, fbar_out, mask, force_field,
-                                          tau_field, tau_bar, force_bar_field, work);
+  if (d->storage == LBM_F32_DEV) MLBM_ADJ(Lattice<9>, f32dev_t);
+  if (d->storage == LBM_F32) MLBM_ADJ(Lattice<9>, float);
+  MLBM_ADJ(Lattice<9>, double);
+#undef MLBM_ADJ
 }
 
 int lbm_moments_adjoint(void* stream, const lbm_desc* d, const void* f, const uint8_t* mask,

@@ -228,12 +228,12 @@ __device__ __forceinline__ void gather_state(const T* src, const uint8_t* mask, 
       d[q] = St::template load<L>(src + q * P + s, q);
       const uint8_t m = (q == 0) ? uint8_t(0) : mask[s];
       if (m) {
+        d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q);
         // 2 w_q (e_q . u_wall) / cs2: momentum a moving wall (mask 2) adds to
         // the population it reflects into direction q (wall density 1)
-        const C lid = m == 2 ? C(6) * weight<L, C>(q) *
-                                   (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c)
-                             : C(0);
-        d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) + lid;
+        if (m == 2)
+          d[q] += C(6) * weight<L, C>(q) *
+                  (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c);
       }
     } else {
       d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
@@ -243,9 +243,9 @@ __device__ __forceinline__ void gather_state(const T* src, const uint8_t* mask, 
 
 // Halfway bounce-back gather from a link word (see FwdArgs::links): the same
 // values as gather_state<L, T, BB_HALF, true>, with one aligned 4-byte read
-// in place of the cell's own and its 18 neighbours' mask bytes. A wave with
-// no blocked link skips the wall code entirely; a cell next to a moving wall
-// also reads its moving-link word, and no neighbour index is needed.
+// in place of the cell's own and its 18 neighbours' mask bytes, and one load
+// per direction; a cell next to a moving wall also reads its moving-link
+// word, and no neighbour index is needed.
 template <class L, class T>
 __device__ __forceinline__ void gather_links(const T* src, const uint32_t* links_moving,
                                              long long P, const Nbr& n, uint32_t lk,
@@ -260,21 +260,22 @@ __device__ __forceinline__ void gather_links(const T* src, const uint32_t* links
     for (int q = 0; q < L::Q; ++q) d[q] = C(0);
     return;
   }
+  // one load per direction: the streamed value, or on a blocked link the
+  // cell's own reflected value (no load is spent on the wall's population)
 #pragma unroll
-  for (int q = 0; q < L::Q; ++q) d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
-  if (lk) {
-    const uint32_t mv = (lk & kLinkMoving) ? links_moving[n.cell] : 0u;
+  for (int q = 0; q < L::Q; ++q) {
+    const bool blocked = (lk >> q) & 1u;  // bit 0 is never set
+    const T* ptr = blocked ? src + L::opp[q] * P + n.cell : src + q * P + n.template at<L>(q, -1);
+    d[q] = St::template load<L>(ptr, q);
+  }
+  if (lk & kLinkMoving) {
+    // 2 w_q (e_q . u_wall) / cs2 on links to a moving wall (density 1)
+    const uint32_t mv = links_moving[n.cell];
 #pragma unroll
-    for (int q = 1; q < L::Q; ++q) {
-      if ((lk >> q) & 1u) {
-        // 2 w_q (e_q . u_wall) / cs2 where the wall moves (density 1)
-        const C lid = ((mv >> q) & 1u) ? C(6) * weight<L, C>(q) *
-                                             (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b +
-                                              C(L::ec[q]) * lid_c)
-                                       : C(0);
-        d[q] = St::template load<L>(src + L::opp[q] * P + n.cell, q) + lid;
-      }
-    }
+    for (int q = 1; q < L::Q; ++q)
+      if ((mv >> q) & 1u)
+        d[q] += C(6) * weight<L, C>(q) *
+                (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c);
   }
 }
 

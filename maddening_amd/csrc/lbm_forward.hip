@@ -543,6 +543,16 @@ int step_push(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
   return LBM_OK;
 }
 
+// Halfway link words (k_links) for the lattice g, two words per cell.
+template <class L, class T>
+struct LinksOp {
+  static int run(hipStream_t s, const Geom& g, const uint8_t* mask, uint32_t* links) {
+    hipLaunchKernelGGL((k_links<L>), dim3(g.nblocks), dim3(g.bx), 0, s, mask, links,
+                       links + g.plane, g.na, g.nb, g.nc, g.bpr);
+    return check_launch("k_links launch");
+  }
+};
+
 template <class L, class T>
 int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in,
                void* f_out, void* work, const uint8_t* mask, const float* ff,
@@ -561,9 +571,7 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
       case LBM_BB_REFERENCE: return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
       default: {
         uint32_t* links = reinterpret_cast<uint32_t*>(static_cast<char*>(work) + links_offset(d, g));
-        hipLaunchKernelGGL((k_links<L>), dim3(g.nblocks), dim3(g.bx), 0, s, mask, links,
-                           links + g.plane, g.na, g.nb, g.nc, g.bpr);
-        if (int rc = check_launch("k_links launch")) return rc;
+        if (int rc = LinksOp<L, T>::run(s, g, mask, links)) return rc;
         a.links = links;
         a.links_moving = links + g.plane;
         return step_pull<L, T, BB_HALF_LINKS>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
@@ -617,13 +625,15 @@ int pull_phase_bb(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T
 template <class L, class T>
 struct PullPhaseOp {
   static int run(hipStream_t s, const lbm_desc* d, const Geom& g, int phase, const void* src,
-                 void* dst, const uint8_t* mask, const float* ff, const float* tf, float* rho,
-                 float* u, float* p) {
+                 void* dst, const uint8_t* mask, const uint32_t* links, const float* ff,
+                 const float* tf, float* rho, float* u, float* p) {
     using C = typename Storage<T>::C;
     FwdArgs<C> a = make_args<C>(d, g);
     a.src = src;
     a.dst = dst;
     a.mask = d->bb_mode == LBM_BB_NONE ? nullptr : mask;
+    a.links = links;
+    a.links_moving = links ? links + g.plane : nullptr;
     a.force_field = ff;
     a.tau_field = tf;
     if (phase == 2) {
@@ -634,7 +644,9 @@ struct PullPhaseOp {
     switch (d->bb_mode) {
       case LBM_BB_NONE: return pull_phase_bb<L, T, BB_NONE>(s, gr, a, phase);
       case LBM_BB_REFERENCE: return pull_phase_bb<L, T, BB_REF>(s, gr, a, phase);
-      default: return pull_phase_bb<L, T, BB_HALF>(s, gr, a, phase);
+      default:
+        if (links) return pull_phase_bb<L, T, BB_HALF_LINKS>(s, gr, a, phase);
+        return pull_phase_bb<L, T, BB_HALF>(s, gr, a, phase);
     }
   }
 };
@@ -758,8 +770,8 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out, voi
 }
 
 int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, void* dst,
-                   const uint8_t* mask, const float* force_field, const float* tau_field,
-                   float* rho, float* u, float* p) {
+                   const uint8_t* mask, const uint32_t* links, const float* force_field,
+                   const float* tau_field, float* rho, float* u, float* p) {
   g_last_error.clear();
   (void)hipGetLastError();
   Geom g;
@@ -770,8 +782,27 @@ int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, 
   if (src == dst) return fail(LBM_ERR_INVALID, "src and dst must be distinct buffers");
   if (d->bb_mode != LBM_BB_NONE && !mask)
     return fail(LBM_ERR_INVALID, "bb_mode needs a wall mask");
-  return dispatch<PullPhaseOp>(d, (hipStream_t)stream, d, g, phase, src, dst, mask, force_field,
-                               tau_field, rho, u, p);
+  if (links && d->bb_mode != LBM_BB_HALFWAY)
+    return fail(LBM_ERR_INVALID, "links are for LBM_BB_HALFWAY only");
+  return dispatch<PullPhaseOp>(d, (hipStream_t)stream, d, g, phase, src, dst, mask, links,
+                               force_field, tau_field, rho, u, p);
+}
+
+size_t lbm_links_size(const lbm_desc* d) {
+  Geom g;
+  if (check_desc(d, &g)) return 0;
+  return 2 * (size_t)g.plane * sizeof(uint32_t);
+}
+
+int lbm_links_build(void* stream, const lbm_desc* d, const uint8_t* mask, uint32_t* links) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (!mask || !links) return fail(LBM_ERR_INVALID, "mask and links must be non-NULL");
+  if (d->q == 19) return LinksOp<Lattice<19>, float>::run((hipStream_t)stream, g, mask, links);
+  return LinksOp<Lattice<9>, float>::run((hipStream_t)stream, g, mask, links);
 }
 
 int lbm_moments(void* stream, const lbm_desc* d, const void* f, const uint8_t* mask,

@@ -43,7 +43,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_string_without_gpu():
     from maddening_amd import _lib
     L = _lib.lib()
-    assert L.lbm_version() == 5
+    assert L.lbm_version() == 6
     assert isinstance(L.lbm_last_error(), bytes)
 
 
@@ -66,7 +66,7 @@ def test_host_validation_errors_without_gpu():
     # exchanged-slot cotangents exist only for the reference wall rule
     d.bb_mode, d.flags = 2, _lib.LBM_FLAG_ADJ_IN_SWAPPED
     assert L.lbm_adjoint_step(None, ctypes.byref(d), None, 1, None, None, None, None, None, None,
-                              None, None) == -2
+                              None, None, None) == -2
     assert b"reference wall rule" in L.lbm_last_error()
     d.bb_mode, d.flags = 0, 0
     assert L.lbm_workspace_size(ctypes.byref(d), 10) == 8 ** 3 * 19 * 4

@@ -222,7 +222,7 @@ def test_exchanged_slot_cotangents_match_plain_chain(lattice, shape, storage):
     f = k.from_aos(f0.to(k.device, k.dtype))
     g1 = k.empty()
     _lib.check(L.lbm_pull_phase(st, ctypes.byref(k.desc()), 0, _ptr(f), _ptr(g1), _ptr(m), None,
-                                _ptr(tf), None, None, None), "pull")
+                                None, _ptr(tf), None, None, None), "pull")
     rho_bar = wts["rho"].float().cuda()
     fbar_N = k.from_aos(wts["f"].to(k.device, k.dtype))
     opp = torch.as_tensor(k.lat.opp, device=k.device).long()
@@ -246,11 +246,11 @@ def test_exchanged_slot_cotangents_match_plain_chain(lattice, shape, storage):
         # step 1 started from f_1 = S B g_1 ... here g_1 stands in for the
         # previous post-collision state; step 0 from f_0 itself
         _lib.check(L.lbm_adjoint_step(st, ctypes.byref(desc(IN | OUT)), _ptr(g1), 1, _ptr(a),
-                                      _ptr(b), _ptr(m), None, _ptr(tf), _ptr(tb), None,
+                                      _ptr(b), _ptr(m), None, None, _ptr(tf), _ptr(tb), None,
                                       _ptr(work)), "step 1")
         mid = b.clone()
         _lib.check(L.lbm_adjoint_step(st, ctypes.byref(desc(IN)), _ptr(f), 0, _ptr(b), _ptr(c),
-                                      _ptr(m), None, _ptr(tf), _ptr(tb), None, _ptr(work)),
+                                      _ptr(m), None, None, _ptr(tf), _ptr(tb), None, _ptr(work)),
                    "step 0")
         out = torch.empty(3, dtype=torch.float64, device=k.device)
         _lib.check(L.lbm_adjoint_end(st, ctypes.byref(desc(0)), _ptr(work), _ptr(out)), "end")
@@ -339,3 +339,38 @@ def test_node_pressure_gradient_matches_finite_differences():
     fd = (float(loss(float(p.detach()) + h, False)) - float(loss(float(p.detach()) - h, False))) \
         / (2 * h)
     assert abs(float(p.grad) - fd) < 1e-6 * abs(fd), (float(p.grad), fd)
+
+
+@pytest.mark.parametrize("lattice,shape,storage", [("D3Q19", (9, 8, 11), "f32"),
+                                                   ("D3Q19", (6, 7, 64), "f64"),
+                                                   ("D2Q9", (21, 33), "f32")])
+def test_halfway_rollout_link_words_match_mask_path(monkeypatch, lattice, shape, storage):
+    """Under halfway bounce-back the rollout builds link words once
+    (lbm_links_build) and every forward, recompute and transposed launch reads
+    them instead of the neighbours' mask bytes; with the mask path instead,
+    the states and all gradients are the same bits."""
+    from maddening_amd import adjoint as adj
+    from maddening_amd.lbm import LBMKernel
+    D = len(shape)
+    f0, mask, tau, force, wts = setup(lattice, shape, "halfway", seed=2)
+    lid = (0.04, -0.03, 0.02)[:D] + (0.0,) * (3 - D)
+    k = LBMKernel(shape, lattice=lattice, tau=0.8, storage=storage, bb="halfway",
+                  lid_velocity=lid)
+    wf = k.from_aos(wts["f"].cuda()).to(k.dtype)
+
+    def run():
+        fg = k.from_aos(f0.cuda().to(k.dtype)).requires_grad_(True)
+        tg = tau.float().cuda().requires_grad_(True)
+        Fg = torch.tensor([1e-5, -2e-5, 3e-5][:D], dtype=torch.float64, device="cuda",
+                          requires_grad=True)
+        fN, rho, u, _ = adj.rollout(k, fg, 9, force=Fg, tau_field=tg, mask=mask.cuda(),
+                                    checkpoint=3)
+        ((fN * wf).sum() + rho.double().sum()).backward()
+        torch.cuda.synchronize()
+        return fN.detach(), fg.grad, tg.grad, Fg.grad
+
+    with_links = run()
+    monkeypatch.setattr(adj, "_links", lambda *a, **kw: None)
+    with_mask = run()
+    for a, b in zip(with_links, with_mask):
+        assert torch.equal(a, b)

@@ -339,8 +339,8 @@ def test_d3q19_halfway_moving_walls_f64(scheme, shape):
 @pytest.mark.parametrize("storage", ["f32dev", "f16dev", "f64"])
 def test_halfway_link_words_match_mask_path_bitwise(lattice, shape, storage):
     """lbm_step's pull sequence reads per-cell link words built once per call;
-    lbm_pull_phase reads the wall mask bytes directly. The same launches in
-    either form give the same bits."""
+    lbm_pull_phase reads the wall mask bytes directly, or the link words of
+    lbm_links_build. The same launches in each form give the same bits."""
     import ctypes
     from maddening_amd import _lib
     from maddening_amd.lbm import _ptr, _stream
@@ -360,21 +360,24 @@ def test_halfway_link_words_match_mask_path_bitwise(lattice, shape, storage):
     L = _lib.lib()
     st = _stream(k.device)
     d = k.desc(force=force)
-    a, b = k.empty(), k.empty()
-    rho = torch.empty(k.grid_shape, dtype=torch.float32, device="cuda")
-    u = torch.empty(k.grid_shape + (D,), dtype=torch.float32, device="cuda")
-    _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 0, _ptr(f), _ptr(a), _ptr(mask), None, None,
-                                None, None, None), "phase 0")
-    for _ in range(nsteps - 1):
-        _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 1, _ptr(a), _ptr(b), _ptr(mask), None,
-                                    None, None, None, None), "phase 1")
-        a, b = b, a
-    out = k.empty()
-    _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 2, _ptr(a), _ptr(out), _ptr(mask), None,
-                                None, _ptr(rho), _ptr(u), None), "phase 2")
-    torch.cuda.synchronize()
-    assert torch.equal(f_step.view(torch.uint8), out.view(torch.uint8))
-    assert torch.equal(rho_s, rho) and torch.equal(u_s, u)
+    links = torch.empty(L.lbm_links_size(ctypes.byref(d)) // 4, dtype=torch.int32, device="cuda")
+    _lib.check(L.lbm_links_build(st, ctypes.byref(d), _ptr(mask), _ptr(links)), "links")
+    for lk in (None, links):
+        a, b = k.empty(), k.empty()
+        rho = torch.empty(k.grid_shape, dtype=torch.float32, device="cuda")
+        u = torch.empty(k.grid_shape + (D,), dtype=torch.float32, device="cuda")
+        _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 0, _ptr(f), _ptr(a), _ptr(mask), _ptr(lk),
+                                    None, None, None, None, None), "phase 0")
+        for _ in range(nsteps - 1):
+            _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 1, _ptr(a), _ptr(b), _ptr(mask),
+                                        _ptr(lk), None, None, None, None, None), "phase 1")
+            a, b = b, a
+        out = k.empty()
+        _lib.check(L.lbm_pull_phase(st, ctypes.byref(d), 2, _ptr(a), _ptr(out), _ptr(mask),
+                                    _ptr(lk), None, None, _ptr(rho), _ptr(u), None), "phase 2")
+        torch.cuda.synchronize()
+        assert torch.equal(f_step.view(torch.uint8), out.view(torch.uint8))
+        assert torch.equal(rho_s, rho) and torch.equal(u_s, u)
 
 
 @pytest.mark.parametrize("lattice,shape", [("D3Q19", (9, 11, 161)), ("D3Q19", (5, 6, 320)),
