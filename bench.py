@@ -313,6 +313,40 @@ def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=16, reps=3):
         res[storage] = rec
         del k, f0, work, fN
         torch.cuda.empty_cache()
+    # the same channel under halfway bounce-back (the rollout reads link
+    # words), float32, timing only
+    k = LBMKernel(shape, tau=0.7, storage="f32", bb="halfway", device=dev)
+    f0 = k.equilibrium()
+    work = k.workspace(nsteps)
+    k.step(f0, 10, mask=mask, force=[0, 0, F0], tau_field=tau, work=work)
+    fwd_t, adj_t = [], []
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    for r in range(reps + 1):
+        e0.record()
+        k.step(f0, nsteps, mask=mask, force=[0, 0, F0], tau_field=tau, work=work)
+        e1.record()
+        torch.cuda.synchronize()
+        Fh = torch.tensor([0.0, 0.0, F0], dtype=torch.float64, device=dev, requires_grad=True)
+        th = tau.clone().requires_grad_(True)
+        fH, _, _, _ = rollout(k, f0, nsteps, force=Fh, tau_field=th, mask=mask)
+        L = uz_loss(k, fH, Fh[2])
+        e2.record()
+        torch.cuda.synchronize()
+        e0b = torch.cuda.Event(enable_timing=True)
+        e0b.record()
+        L.backward()
+        e2.record()
+        torch.cuda.synchronize()
+        if r:  # the first round warms the allocator
+            fwd_t.append(e0.elapsed_time(e1) / nsteps)
+            adj_t.append(e0b.elapsed_time(e2) / nsteps)
+        del fH, Fh, th, L
+    fwd_ms, adj_ms = statistics.median(fwd_t), statistics.median(adj_t)
+    res["halfway_f32"] = {"forward_ms_per_step": round(fwd_ms, 4),
+                          "adjoint_ms_per_step": round(adj_ms, 4),
+                          "adjoint_over_forward": round(adj_ms / fwd_ms, 3), "reps": reps}
+    del k, f0, work
+    torch.cuda.empty_cache()
     return res
 
 
