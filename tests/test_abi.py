@@ -77,6 +77,14 @@ def test_host_validation_errors_without_gpu():
     assert L.lbm_workspace_size(ctypes.byref(d), 10) == 8 ** 3 * 19 * 4 + 8 ** 3 * 8
     d.storage, d.dims[0], d.dims[1], d.dims[2] = 2, 3, 5, 7
     assert L.lbm_workspace_size(ctypes.byref(d), 10) == 4096 + 105 * 8
+    assert L.lbm_links_size(ctypes.byref(d)) == 105 * 8
+    # link words are for halfway bounce-back only
+    d.bb_mode = 1
+    assert L.lbm_pull_phase(None, ctypes.byref(d), 1, ctypes.c_void_p(4096), ctypes.c_void_p(8192),
+                            ctypes.c_void_p(4096), ctypes.c_void_p(4096), None, None, None, None,
+                            None) == -1
+    assert b"LBM_BB_HALFWAY only" in L.lbm_last_error()
+    assert L.lbm_links_build(None, ctypes.byref(d), None, None) == -1
 
 
 def test_batch_item_validation_without_gpu():
