@@ -440,3 +440,36 @@ def test_xcd_run_orders_match_bitwise(shape, bx):
     for other in outs[1:]:
         for a, b in zip(outs[0], other):
             assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("lattice,shape", [("D3Q19", (7, 9, 11)), ("D3Q19", (1, 5, 3)),
+                                           ("D2Q9", (13, 17)), ("D2Q9", (1, 6))])
+def test_link_words_equal_brute_force_link_sets(lattice, shape):
+    """lbm_links_build against a NumPy brute force over the mask, word for
+    word: bit q when x - e_q (periodic) is a wall, bit 30 when one of those
+    walls moves (mask 2), bit 31 alone for a wall cell; second word: the links
+    that end at a moving wall."""
+    import ctypes
+    from maddening_amd import _lib
+    from maddening_amd.lbm import _ptr, _stream
+    rng = np.random.default_rng(sum(shape))
+    m = np.where(rng.random(shape) < 0.2, 1, np.where(rng.random(shape) < 0.15, 2, 0)).astype(np.uint8)
+    k = kernel(shape, lattice, 0.8, bb="halfway")
+    L = _lib.lib()
+    d = k.desc()
+    links = torch.empty(L.lbm_links_size(ctypes.byref(d)) // 4, dtype=torch.int32, device="cuda")
+    _lib.check(L.lbm_links_build(_stream(k.device), ctypes.byref(d),
+                                 _ptr(torch.from_numpy(m).cuda()), _ptr(links)), "links")
+    got = links.cpu().numpy().view(np.uint32).reshape((2,) + tuple(shape))
+    e, _, _ = orc.lattice(k.Q)
+    w0 = np.zeros(shape, np.uint32)
+    w1 = np.zeros(shape, np.uint32)
+    for q in range(1, k.Q):
+        src = np.roll(m, shift=tuple(int(v) for v in e[q]), axis=tuple(range(len(shape))))
+        w0 |= np.where(src != 0, np.uint32(1 << q), np.uint32(0))
+        w1 |= np.where(src == 2, np.uint32(1 << q), np.uint32(0))
+    w0 |= np.where(w1 != 0, np.uint32(1 << 30), np.uint32(0))
+    w0 = np.where(m != 0, np.uint32(1 << 31), w0)
+    w1 = np.where(m != 0, np.uint32(0), w1)
+    assert np.array_equal(got[0], w0)
+    assert np.array_equal(got[1], w1)
