@@ -1,7 +1,8 @@
 """Launch-shape autotuner with an on-disk cache.
 
-For a single lattice it times the one-cell-per-thread kernels at each
-workgroup width and the row-per-wave kernels at each legal vector width;
+For a single lattice it times the library's own launch choice, the
+one-cell-per-thread kernels at each workgroup width, flattened tiles, the
+row-per-wave kernels at each legal vector width and XCD run lengths;
 for a batch it times tile sizes (cells per thread) against persistent grid
 sizes. Each candidate runs a few warm steps, then a timed short run; the
 fastest is stored in a JSON cache keyed by device name and problem shape
@@ -59,35 +60,59 @@ def _time(fn, reps: int) -> float:
     return best
 
 
+_KNOBS = ("block_x", "row_vec", "scheme", "xcd_remap", "xcd_group")
+
+
+def _apply(kern: LBMKernel, cand: dict) -> None:
+    for k in _KNOBS:
+        setattr(kern, k, cand[k])
+
+
 def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool = True) -> dict:
-    """Pick block_x / row_vec / scheme for ``kern``'s lattice; applies and
-    returns the winner with every candidate's ms per step."""
+    """Pick the launch shape for ``kern``'s lattice: the library's own choice,
+    row blocks of each width (pull or push), flattened 256-cell tiles, the
+    row-per-wave kernels at each legal vector width, then XCD run lengths for
+    the best of those. Applies and returns the winner with every candidate's
+    ms per step."""
     dev = torch.cuda.get_device_name(kern.device)
-    key = f"kernel|{dev}|{kern.lat.name}|{kern.grid_shape}|{kern.storage}|{kern.bb}"
+    key = f"kernel2|{dev}|{kern.lat.name}|{kern.grid_shape}|{kern.storage}|{kern.bb}"
     if use_cache:
         hit = _load().get(key)
         if hit:
-            kern.block_x, kern.row_vec, kern.scheme = hit["block_x"], hit["row_vec"], hit["scheme"]
+            _apply(kern, hit)
             return hit
-    cands = [(bx, -1, sc) for bx in (64, 128, 256) for sc in ("pull", "push")]
+
+    def cand(block_x=0, row_vec=0, scheme="auto", xcd_remap=False, xcd_group=0):
+        return {"block_x": block_x, "row_vec": row_vec, "scheme": scheme,
+                "xcd_remap": xcd_remap, "xcd_group": xcd_group}
+
+    shapes = [cand()]
+    shapes += [cand(bx, -1, sc) for bx in (64, 128, 256) for sc in ("pull", "push")]
+    shapes.append(cand(-1, -1, "pull"))
     if kern.bb == "none":
-        cands += [(0, v, "pull") for v in (1, 2, 4)
-                  if kern.grid_shape[-1] % (64 * v) == 0 and not (v == 4 and kern.storage == "f64")]
+        shapes += [cand(0, v, "pull") for v in (1, 2, 4)
+                   if kern.grid_shape[-1] % (64 * v) == 0 and not (v == 4 and kern.storage == "f64")]
     f = kern.equilibrium()
     out = kern.empty()
     mask = torch.zeros(kern.grid_shape, dtype=torch.uint8, device=kern.device) \
         if kern.bb != "none" else None
-    saved = (kern.block_x, kern.row_vec, kern.scheme)
-    results = []
-    for bx, rv, sc in cands:
-        kern.block_x, kern.row_vec, kern.scheme = bx, rv, sc
+    saved = {k: getattr(kern, k) for k in _KNOBS}
+
+    def measure(c):
+        _apply(kern, c)
         work = kern.workspace(nsteps)
         ms = _time(lambda: kern.step(f, nsteps, mask=mask, out=out, work=work, moments=False),
                    reps) / nsteps
-        results.append({"block_x": bx, "row_vec": rv, "scheme": sc, "ms_per_step": ms})
-    kern.block_x, kern.row_vec, kern.scheme = saved
+        return dict(c, ms_per_step=ms)
+
+    results = [measure(c) for c in shapes]
     best = min(results, key=lambda r: r["ms_per_step"])
-    kern.block_x, kern.row_vec, kern.scheme = best["block_x"], best["row_vec"], best["scheme"]
+    # XCD run lengths (1 is the identity order) on the best shape
+    results += [measure(dict({k: best[k] for k in _KNOBS}, xcd_remap=True, xcd_group=g))
+                for g in (1, 8, 64, 512)]
+    _apply(kern, saved)
+    best = min(results, key=lambda r: r["ms_per_step"])
+    _apply(kern, best)
     rec = dict(best, candidates=results)
     _store(key, rec)
     return rec

@@ -473,3 +473,26 @@ def test_link_words_equal_brute_force_link_sets(lattice, shape):
     w1 = np.where(m != 0, np.uint32(0), w1)
     assert np.array_equal(got[0], w0)
     assert np.array_equal(got[1], w1)
+
+
+def test_tune_kernel_picks_a_candidate_and_keeps_results(tmp_path, monkeypatch):
+    """The single-lattice tuner times every launch shape, applies the fastest
+    and caches it; the tuned kernel gives the same bits as the default one."""
+    from maddening_amd import autotune
+    monkeypatch.setattr(autotune, "_CACHE", str(tmp_path / "tune.json"))
+    shape = (12, 10, 40)
+    k = kernel(shape, "D3Q19", 0.8, bb="reference")
+    rec = autotune.tune_kernel(k, nsteps=3, reps=1)
+    assert len(rec["candidates"]) >= 12
+    assert rec["ms_per_step"] == min(c["ms_per_step"] for c in rec["candidates"])
+    assert (k.block_x, k.row_vec, k.scheme, k.xcd_remap, k.xcd_group) == \
+        (rec["block_x"], rec["row_vec"], rec["scheme"], rec["xcd_remap"], rec["xcd_group"])
+    assert autotune.tune_kernel(kernel(shape, "D3Q19", 0.8, bb="reference"))["ms_per_step"] == \
+        rec["ms_per_step"]  # served from the cache
+    rng = np.random.default_rng(0)
+    mask = torch.from_numpy((rng.random(shape) < 0.2).astype(np.uint8)).cuda()
+    f0 = kernel(shape, "D3Q19", 0.8, bb="reference").equilibrium()
+    ref = kernel(shape, "D3Q19", 0.8, bb="reference", scheme="pull").step(f0, 6, mask=mask)[0]
+    k.scheme = "pull"
+    got = k.step(f0, 6, mask=mask)[0]
+    assert torch.equal(got, ref)
