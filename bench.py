@@ -29,8 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 STORAGE_BYTES = {"f32": 4, "f32dev": 4, "f64": 8, "f16dev": 2, "bf16dev": 2}
 
 
-def fused_kernel_label(storage, row_vec):
+def fused_kernel_label(storage, row_vec, pairs=0):
     """Name of the kernel that runs the fused pull+collide launches."""
+    if pairs:
+        return f"k_pair<D3Q19,{storage}> (two steps per launch)"
     if row_vec > 0:
         return f"k_row<D3Q19,{storage},V={row_vec}>"
     return f"k_cell<D3Q19,{storage},pull+collide>"
@@ -82,8 +84,40 @@ def time_kernel(kern, f0, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev = start.elapsed_time(end) / 1e3
-    launches = steps + 1  # pull: collide + (K-1) fused + stream; push: K + moments pass
-    return ev, wall, launches, out
+    # pull: collide, two-step launches, single fused launches, stream + moments
+    launches, pairs = kern.schedule(steps)
+    return ev, wall, launches, pairs, out
+
+
+def time_pair_launch(kern, f0, reps=40):
+    """Average duration of one two-step launch (lbm_pull_phase phase 3, the
+    kernel lbm_step runs for pairs of steps), timed with HIP events on the
+    stream the library launches on (torch's current stream)."""
+    import ctypes
+    import torch
+    from maddening_amd import _lib
+    from maddening_amd.lbm import _stream
+    d = kern.desc()
+    L = _lib.lib()
+    s = _stream(kern.device)
+    a, b = f0.clone(), kern.empty()
+
+    def launch(src, dst):
+        _lib.check(L.lbm_pull_phase(s, ctypes.byref(d), 3, ctypes.c_void_p(src.data_ptr()),
+                                    ctypes.c_void_p(dst.data_ptr()), None, None, None, None,
+                                    None, None, None), "lbm_pull_phase")
+
+    for i in range(4):
+        launch(a, b) if i % 2 == 0 else launch(b, a)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for i in range(reps):
+        launch(a, b) if i % 2 == 0 else launch(b, a)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / 1e3 / reps
 
 
 def _cpu_model():
@@ -478,10 +512,12 @@ def pipe_leg(dev, shape=(256, 128, 128), nsteps=200, reps=3):
     return out
 
 
-def read_traffic(storage):
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", f"pmc_d3q19_256_{storage}.json")
+def read_traffic(storage, pair=False):
+    """HBM bytes per launch of the fused kernel (pair: of the two-step kernel)
+    from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled
+    by the copy-kernel factor)."""
+    name = f"pmc_d3q19_256_{storage}_pair.json" if pair else f"pmc_d3q19_256_{storage}.json"
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as fh:
             return json.load(fh).get("traffic_bytes_per_launch")
@@ -512,6 +548,8 @@ def main():
                          "100/1000/5000 steps), adjoint (forward+adjoint ms/step, gradient "
                          "vs central differences), batch (64 uneven lattices), pipe "
                          "(LBMPipeNode single phase and multiphase), d2q9 (8192^2 D2Q9)")
+    ap.add_argument("--pair", default="auto", choices=["auto", "off", "force"],
+                    help="two pull steps per launch (LBM_FLAG_PAIR_*)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
@@ -533,7 +571,7 @@ def main():
     def make(storage):
         return LBMKernel((n, n, n), lattice="D3Q19", tau=0.8, storage=storage,
                          scheme=args.scheme, block_x=args.block_x, row_vec=args.row_vec,
-                         xcd_remap=args.xcd_remap, device=dev)
+                         xcd_remap=args.xcd_remap, pair=args.pair, device=dev)
 
     if args.calibrate_copy:
         a = torch.ones(1 << 28, device=dev)  # 1 GiB float32
@@ -545,12 +583,19 @@ def main():
 
     kern = make(args.storage)
     f0 = kern.equilibrium(rho, u)
-    ev, wall, launches, out = time_kernel(kern, f0, args.steps, args.warmup)
+    ev, wall, launches, pairs, out = time_kernel(kern, f0, args.steps, args.warmup)
     mlups = cells * args.steps / ev / 1e6
     b_cell = 2 * 19 * STORAGE_BYTES[args.storage]
     rv = auto_row_vec(args.storage, n, args.row_vec) if kern.scheme != "push" else 0
-    per_launch_s = ev / launches
+    if pairs:
+        # dominant kernel: the two-step launch, timed on its own. Per launch it
+        # must read the state once and write it once (b_cell bytes per cell)
+        # while it performs two cell updates per cell.
+        per_launch_s = time_pair_launch(kern, f0)
+    else:
+        per_launch_s = ev / launches
     achieved = b_cell * cells / per_launch_s / 1e9
+    effective = mlups * 1e6 * b_cell / 1e9
     result = {
         "metric": "D3Q19 MLUPS & %HBM peak (fp32 vs fp16-dev storage); fwd+adjoint ms/step",
         "value": round(mlups, 1),
@@ -566,30 +611,64 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"D3Q19 BGK periodic {n}^3 Taylor-Green vortex, u0=0.05, "
                                f"tau=0.8, populations stored as {args.storage}",
-                   "cells": cells, "scheme": kern.scheme, "wall_s_timed_call": round(wall, 4)},
+                   "cells": cells, "scheme": kern.scheme, "launches": launches,
+                   "two_step_launches": pairs, "wall_s_timed_call": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(args.storage),
-                     "kernel": f"{fused_kernel_label(args.storage, rv)} "
-                               f"({b_cell} B/cell algorithmic)",
-                     "avg_launch_us": round(per_launch_s * 1e6, 2)},
+                     "traffic": read_traffic(args.storage, pair=bool(pairs)),
+                     "kernel": f"{fused_kernel_label(args.storage, rv, pairs)} "
+                               f"({b_cell} B/cell per launch algorithmic)",
+                     "avg_launch_us": round(per_launch_s * 1e6, 2),
+                     "updates_per_launch": 2 if pairs else 1,
+                     # MLUPS x b_cell: the HBM rate a one-update-per-pass step
+                     # would need for this update rate
+                     "effective_GBs": round(effective, 1),
+                     "effective_frac": round(effective / HBM_PEAK_GBS, 4)},
     }
+    if pairs:
+        # the same workload with one update per launch (LBM_FLAG_PAIR_OFF)
+        ks = LBMKernel((n, n, n), lattice="D3Q19", tau=0.8, storage=args.storage,
+                       scheme=args.scheme, block_x=args.block_x, row_vec=args.row_vec,
+                       xcd_remap=args.xcd_remap, pair="off", device=dev)
+        ev1, _, launches1, _, out1 = time_kernel(ks, f0, args.steps, args.warmup)
+        t1 = ev1 / launches1
+        result["single_pass"] = {
+            "MLUPS": round(cells * args.steps / ev1 / 1e6, 1),
+            "kernel": f"{fused_kernel_label(args.storage, rv)}",
+            "avg_launch_us": round(t1 * 1e6, 2),
+            "achieved_GBs": round(b_cell * cells / t1 / 1e9, 1),
+            "frac": round(b_cell * cells / t1 / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(args.storage),
+            "same_bits_as_two_step": bool(torch.equal(out1, out))}
+        del out1, ks
     extra = {}
     for storage in [s for s in args.variants.split(",") if s]:
         k2 = make(storage)
         f2 = k2.equilibrium(rho, u)
-        ev2, _, launches2, _ = time_kernel(k2, f2, args.steps, args.warmup)
+        ev2, _, launches2, pairs2, _ = time_kernel(k2, f2, args.steps, args.warmup)
         bc2 = 2 * 19 * STORAGE_BYTES[storage]
         extra[storage] = {"MLUPS": round(cells * args.steps / ev2 / 1e6, 1),
-                          "GB/s": round(bc2 * cells / (ev2 / launches2) / 1e9, 1),
+                          "effective_GBs": round(bc2 * cells * args.steps / ev2 / 1e9, 1),
+                          "two_step_launches": pairs2,
                           "speedup_vs_headline": round(ev / ev2, 3), "_ev": ev2}
+        if pairs2:
+            ks = LBMKernel((n, n, n), lattice="D3Q19", tau=0.8, storage=storage,
+                           scheme=args.scheme, pair="off", device=dev)
+            ev3, _, _, _, _ = time_kernel(ks, f2, args.steps, args.warmup)
+            extra[storage]["single_pass_MLUPS"] = round(cells * args.steps / ev3 / 1e6, 1)
+            extra[storage]["_ev1"] = ev3
+            del ks
         del f2, k2
     ref32 = extra["f32"]["_ev"] if "f32" in extra else (ev if args.storage == "f32" else None)
+    ref32_1 = extra["f32"].get("_ev1") if "f32" in extra else None
     for v in extra.values():
+        ev_v = v.pop("_ev")
+        v.pop("_ev1", None)
         if ref32 is not None:
-            v["speedup_vs_f32"] = round(ref32 / v.pop("_ev"), 3)
-        else:
-            v.pop("_ev")
+            v["speedup_vs_f32"] = round(ref32 / ev_v, 3)
+        if ref32_1 is not None:
+            # against float32 storage with one update per launch
+            v["speedup_vs_f32_single_pass"] = round(ref32_1 / ev_v, 3)
     if extra:
         result["variants"] = extra
     del out, f0

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LBM_API_VERSION 6
+#define LBM_API_VERSION 7
 
 typedef enum lbm_status {
   LBM_OK = 0,
@@ -115,6 +115,16 @@ typedef struct lbm_desc {
  * each face is added to work. Set it on every lbm_adjoint_step of a chain
  * except the one that returns dL/df_0, and on lbm_moments_adjoint. */
 #define LBM_FLAG_ADJ_OUT_FACES 8
+/* Two steps per launch (no counterpart in the reference: a launch schedule
+ * of the same update). lbm_step runs pairs of its fused pull steps as one
+ * launch that keeps the intermediate state in LDS, where the lattice allows
+ * it: D3Q19, LBM_BB_NONE, LBM_F32 or LBM_F32_DEV, uniform force and tau, no
+ * pressure faces, dims[2] of 64, 128 or 256 and 19 * cells < 2^31. By
+ * default only where it measured faster (dims[2] == 256 and enough
+ * workgroups to fill the GPU); PAIR_FORCE uses it wherever it applies,
+ * PAIR_OFF never. Results are bit-identical either way. */
+#define LBM_FLAG_PAIR_OFF 16
+#define LBM_FLAG_PAIR_FORCE 32
 
 /* ---- queries ----------------------------------------------------------- */
 
@@ -160,7 +170,10 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out,
  *   phase 1: src = post-collision g_{n-1}, dst = C(f_n) with f_n gathered
  *            from g_{n-1} by streaming and bounce-back;
  *   phase 2: src = post-collision g_{n-1}, dst = f_n, and the moments of f_n
- *            into rho/u/p (each nullable).
+ *            into rho/u/p (each nullable);
+ *   phase 3: src = post-collision g_{n-1}, dst = C(f_{n+1}): two phase-1
+ *            launches in one, bit for bit (LBM_FLAG_PAIR_*); returns
+ *            LBM_ERR_UNSUPPORTED where the two-step launch does not apply.
  * links (nullable, LBM_BB_HALFWAY only): the link words lbm_links_build made
  * from this mask; the gathers then read one word per cell instead of the
  * neighbours' mask bytes (the same results).
@@ -168,6 +181,17 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out,
 int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, void* dst,
                    const uint8_t* mask, const uint32_t* links, const float* force_field,
                    const float* tau_field, float* rho, float* u, float* p);
+
+/*
+ * Launches lbm_step makes for nsteps steps with this descriptor (with_fields:
+ * the call passes a force or tau field), and how many of them are two-step
+ * launches (LBM_FLAG_PAIR_*). Pull sequence: collide, two-step launches,
+ * single fused launches, stream + moments. Push scheme: one launch per step
+ * plus the moments pass (Zou-He faces add one launch per face and step).
+ * No counterpart in the reference; for timing and profiling.
+ */
+int lbm_step_schedule(const lbm_desc* d, int nsteps, int with_fields, int* launches,
+                      int* pair_launches);
 
 /*
  * Link words of a wall mask for LBM_BB_HALFWAY (no counterpart in the

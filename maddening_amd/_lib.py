@@ -21,6 +21,8 @@ LBM_FLAG_XCD_REMAP = 1
 LBM_FLAG_ADJ_IN_SWAPPED = 2
 LBM_FLAG_ADJ_OUT_SWAPPED = 4
 LBM_FLAG_ADJ_OUT_FACES = 8
+LBM_FLAG_PAIR_OFF = 16
+LBM_FLAG_PAIR_FORCE = 32
 
 STORAGE_NAMES = {"f32": LBM_F32, "f64": LBM_F64, "f16dev": LBM_F16_DEV,
                  "bf16dev": LBM_BF16_DEV, "f32dev": LBM_F32_DEV}
@@ -119,6 +121,9 @@ _PROTOS = {
                                         _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lbm_pull_phase": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), ctypes.c_int, _vp, _vp, _vp,
                                       _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lbm_step_schedule": (ctypes.c_int, [ctypes.POINTER(LbmDesc), ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]),
     "lbm_links_size": (ctypes.c_size_t, [ctypes.POINTER(LbmDesc)]),
     "lbm_links_build": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),
     "lbm_moments_adjoint": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp, _vp, _vp,

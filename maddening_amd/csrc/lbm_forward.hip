@@ -26,6 +26,7 @@
 #include "lbm_common.hpp"
 #include "lbm_host.hpp"
 #include "lbm_cell.hpp"
+#include "lbm_pair.hpp"
 
 namespace mlbm {
 
@@ -485,24 +486,117 @@ int launch_push(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>:
   return check_launch("k_push launch");
 }
 
-// Pull sequence: collide (f_in -> B0), fused pull+collide (B -> B'), final
-// pull + moments (B -> f_out). The ping-pong parity puts the last buffer
-// before f_out in `work`.
+// ---------------------------------------------------------------------------
+// Two steps per launch (k_pair, lbm_pair.hpp)
+// ---------------------------------------------------------------------------
+
+// Storage types k_pair is built for: 32-bit storage. With 16-bit storage it
+// measured slower than the row kernel (256^3: 569 us per two steps against
+// 2 x 215 us), because it is bound by load latency rather than by bytes.
+template <class T>
+constexpr bool pair_storage() {
+  return std::is_same<T, float>::value || std::is_same<T, f32dev_t>::value;
+}
+
+// Plan of the two-step launch for lattice g, or false where it does not
+// apply: D3Q19, periodic, 32-bit storage, uniform force and tau (`fields`:
+// the call passes a force or tau field), no pressure faces, rows of 64, 128
+// or 256 cells, and 19 planes addressable with 32-bit offsets.
+// Without LBM_FLAG_PAIR_FORCE it is used only where it was measured faster
+// than two fused launches: rows of 256 cells, chunks of at least 64 planes
+// and at least 256 workgroups, one per CU (256^3 f32dev: 626 us per launch
+// against 910 us for two k_cell launches, bit-identical results). With rows
+// of 128 cells and 22-43 workgroups it measured 2-4x slower.
+inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
+  if (d->flags & LBM_FLAG_PAIR_OFF) return false;
+  const bool force = (d->flags & LBM_FLAG_PAIR_FORCE) != 0;
+  if (d->q != 19 || d->bb_mode != LBM_BB_NONE || fields) return false;
+  if (d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
+  if (d->inlet_face > 0 || d->outlet_face > 0) return false;
+  if (g.nc != 64 && g.nc != 128 && g.nc != 256) return false;
+  if (19LL * g.plane >= (1LL << 31)) return false;
+  if (!force && g.nc != 256) return false;
+  const int nc_log2 = g.nc == 64 ? 6 : (g.nc == 128 ? 7 : 8);
+  const int tb = (1024 >> nc_log2) - 2;
+  const long long btiles = (g.nb + tb - 1) / tb;
+  int la = 128;
+  auto blocks = [&](int l) { return btiles * ((g.na + l - 1) / l); };
+  while (!force && blocks(la) < 256 && la > 64) la /= 2;
+  if (!force && blocks(la) < 256) return false;
+  if (blocks(la) >= (1LL << 31)) return false;
+  std::memset(pa, 0, sizeof(*pa));
+  pa->na = g.na;
+  pa->nb = g.nb;
+  pa->nc = g.nc;
+  pa->plane = (int)g.plane;
+  pa->nc_log2 = nc_log2;
+  pa->rows_log2 = 10 - nc_log2;
+  pa->tb = tb;
+  pa->btiles = (int)btiles;
+  pa->la = la;
+  pa->nblocks = (int)blocks(la);
+  // runs of 16 workgroups per XCD (256^3: 646 us against 695 us identity)
+  pa->remap = 16;
+  if (d->flags & LBM_FLAG_XCD_REMAP) pa->remap = d->xcd_group > 0 ? d->xcd_group : pa->nblocks / 8;
+  pa->inv_tau = (float)(1.0 / d->tau);
+  pa->Fa = (float)d->force[0];
+  pa->Fb = (float)d->force[1];
+  pa->Fc = (float)d->force[2];
+  return true;
+}
+
+template <class T>
+int launch_pair(hipStream_t s, const PairArgs& pa) {
+  hipLaunchKernelGGL((k_pair<T>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s, pa);
+  return check_launch("k_pair launch");
+}
+
+// Launches of a pull sequence of nsteps steps: collide, npair two-step
+// launches, nsingle fused launches, stream + moments.
+struct PullSchedule {
+  int npair, nsingle, launches;
+};
+inline PullSchedule pull_schedule(int nsteps, bool pair) {
+  PullSchedule sc;
+  const int fused = nsteps - 1;
+  sc.npair = pair ? fused / 2 : 0;
+  sc.nsingle = fused - 2 * sc.npair;
+  sc.launches = 2 + sc.npair + sc.nsingle;
+  return sc;
+}
+
+// Pull sequence: collide (f_in -> B0), fused pull+collide (B -> B'), two
+// fused steps per launch where `pair` is given, final pull + moments
+// (B -> f_out). Launch j writes buf[(launches - 1 - j) & 1], so the last
+// writes f_out and reads `work`.
 template <class L, class T, int BB>
 int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
               const void* f_in, void* f_out, void* work, int nsteps,
-              float* rho, float* u, float* p) {
+              float* rho, float* u, float* p, const PairArgs* pair = nullptr) {
   using C = typename Storage<T>::C;
   int rc;
-  void* buf[2] = {work, f_out};  // buf[(nsteps-1-i) & 1] holds the state after kernel i
+  const PullSchedule sc = pull_schedule(nsteps, pair != nullptr);
+  void* buf[2] = {f_out, work};
+  int j = 0;
   FwdArgs<C> k = a;
   k.rho = k.u = k.p = nullptr;
   k.src = f_in;
-  k.dst = buf[(nsteps - 1) & 1];
+  k.dst = buf[(sc.launches - 1 - j++) & 1];
   if ((rc = launch_cell<L, T, BB, false, true, true>(s, g, k))) return rc;
-  for (int i = 1; i < nsteps; ++i) {
-    k.src = buf[(nsteps - i) & 1];
-    k.dst = buf[(nsteps - 1 - i) & 1];
+  if constexpr (L::Q == 19 && BB == BB_NONE && pair_storage<T>()) {
+    if (sc.npair) {
+      PairArgs pa = *pair;
+      for (int i = 0; i < sc.npair; ++i) {
+        pa.src = k.dst;
+        pa.dst = buf[(sc.launches - 1 - j++) & 1];
+        if ((rc = launch_pair<T>(s, pa))) return rc;
+        k.dst = pa.dst;
+      }
+    }
+  }
+  for (int i = 0; i < sc.nsingle; ++i) {
+    k.src = k.dst;
+    k.dst = buf[(sc.launches - 1 - j++) & 1];
     if ((rc = launch_cell<L, T, BB, true, true, true>(s, g, k))) return rc;
   }
   k.src = work;
@@ -567,7 +661,12 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
   gr.rowv = scheme == LBM_SCHEME_PULL ? row_width(d, g) : 0;
   if (scheme == LBM_SCHEME_PULL) {
     switch (d->bb_mode) {
-      case LBM_BB_NONE: return step_pull<L, T, BB_NONE>(s, gr, a, f_in, f_out, work, nsteps, rho, u, p);
+      case LBM_BB_NONE: {
+        PairArgs pa;
+        const bool pair = pair_storage<T>() && pair_plan(d, g, ff || tf, &pa);
+        return step_pull<L, T, BB_NONE>(s, gr, a, f_in, f_out, work, nsteps, rho, u, p,
+                                        pair ? &pa : nullptr);
+      }
       case LBM_BB_REFERENCE: return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
       default: {
         uint32_t* links = reinterpret_cast<uint32_t*>(static_cast<char*>(work) + links_offset(d, g));
@@ -638,6 +737,17 @@ struct PullPhaseOp {
     a.tau_field = tf;
     if (phase == 2) {
       a.rho = rho; a.u = u; a.p = p;
+    }
+    if (phase == 3) {
+      PairArgs pa;
+      if (!pair_storage<T>() || !pair_plan(d, g, ff || tf, &pa))
+        return fail(LBM_ERR_UNSUPPORTED,
+                    "phase 3 (two steps per launch) does not apply to this lattice");
+      if constexpr (pair_storage<T>()) {
+        pa.src = src;
+        pa.dst = dst;
+        return launch_pair<T>(s, pa);
+      }
     }
     Geom gr = g;
     gr.rowv = row_width(d, g);
@@ -769,6 +879,27 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out, voi
                           tau_field, rho, u, p, nsteps, scheme);
 }
 
+int lbm_step_schedule(const lbm_desc* d, int nsteps, int with_fields, int* launches,
+                      int* pair_launches) {
+  g_last_error.clear();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (nsteps < 1) return fail(LBM_ERR_INVALID, "nsteps must be >= 1");
+  int nl, np = 0;
+  if (resolve_scheme(d, nsteps) == LBM_SCHEME_PUSH) {
+    nl = nsteps + 1;  // one launch per step, then the moments pass
+  } else {
+    PairArgs pa;
+    const PullSchedule sc = pull_schedule(nsteps, pair_plan(d, g, with_fields != 0, &pa));
+    nl = sc.launches;
+    np = sc.npair;
+  }
+  if (launches) *launches = nl;
+  if (pair_launches) *pair_launches = np;
+  return LBM_OK;
+}
+
 int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, void* dst,
                    const uint8_t* mask, const uint32_t* links, const float* force_field,
                    const float* tau_field, float* rho, float* u, float* p) {
@@ -777,7 +908,7 @@ int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, 
   Geom g;
   int rc = check_desc(d, &g);
   if (rc) return rc;
-  if (phase < 0 || phase > 2) return fail(LBM_ERR_INVALID, "phase must be 0, 1 or 2");
+  if (phase < 0 || phase > 3) return fail(LBM_ERR_INVALID, "phase must be 0, 1, 2 or 3");
   if (!src || !dst) return fail(LBM_ERR_INVALID, "src and dst must be non-NULL");
   if (src == dst) return fail(LBM_ERR_INVALID, "src and dst must be distinct buffers");
   if (d->bb_mode != LBM_BB_NONE && !mask)

@@ -1,0 +1,140 @@
+// Two pull steps per launch for periodic D3Q19 lattices (temporal blocking
+// through LDS). Included by lbm_forward.hip.
+//
+// One fused pull launch reads the post-collision state g_n at x - e_q and
+// writes g_{n+1}(x) = C(f_{n+1}(x)): 152 B of HBM per cell update in float32.
+// k_pair computes g_{n+2} from g_n in one pass, so the state crosses HBM once
+// per two updates. A 1024-thread workgroup owns R = 1024 / nc whole rows of
+// the contiguous axis c (nc in {64, 128, 256}), rows b0 - 1 .. b0 + R - 2 of
+// axis b, and marches along axis a over a chunk of planes:
+//
+//   plane p:  step 1 at (p, row, c) for all R rows: gather g_n from HBM
+//             (the next plane's gather is issued before step 2, so its
+//             latency overlaps step 2's arithmetic and stores), collide;
+//             step 2 at (p - 1, row, c) for the R - 2 inner rows: gather
+//             g_{n+1} from step 1's results, collide, store g_{n+2}.
+//
+// Step 2 at plane p - 1 needs g_{n+1} from plane p - 2 for the five
+// directions with e_a = +1, from plane p - 1 for the nine with e_a = 0 and
+// from plane p for the five with e_a = -1. Each thread keeps its own
+// cell's values of the last two planes in registers. The 16 directions
+// that cross to another thread (all but q = 0, 1, 2, whose source is the
+// thread's own column) pass through 64 KB of LDS, written per plane:
+// e_a = -1 values of plane p, e_a = 0 values of plane p - 1 and e_a = +1
+// values of plane p - 2. Because a workgroup spans whole rows, the periodic
+// wrap along c stays inside it; the wrap along a and b is index arithmetic.
+//
+// Each update is the same inlined moments + collision as k_cell's, and step
+// 1's results pass through the storage type's encode/decode exactly as a
+// store and reload would, so the output equals two k_cell launches bit for
+// bit. Halo rows (the first and last of a workgroup) and the plane before
+// each chunk are computed twice, by two workgroups; that redundant
+// arithmetic and the extra reads it makes are served by the caches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lbm_common.hpp"
+
+namespace mlbm {
+
+struct PairArgs {
+  const void* src;
+  void* dst;
+  int na, nb, nc;  // nc in {64, 128, 256}
+  int plane;       // na * nb * nc; 19 * plane < 2^31, so offsets fit 32 bits
+  int rows_log2;   // log2(1024 / nc): rows per workgroup
+  int nc_log2;
+  int tb;          // output rows per workgroup = 1024 / nc - 2
+  int btiles;      // ceil(nb / tb)
+  int la;          // planes per chunk
+  int nblocks;     // btiles * ceil(na / la)
+  int remap;       // xcd_remap run length (0: identity)
+  float inv_tau, Fa, Fb, Fc;
+};
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
+  using L = Lattice<19>;
+  using St = Storage<T>;
+  using C = float;
+  static_assert(sizeof(typename Storage<T>::C) == 4, "float compute only");
+  __shared__ float x[16][1024];
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
+  const int bt = lb % a.btiles;
+  const int a0 = (lb / a.btiles) * a.la;
+  const int a1 = a0 + a.la < a.na ? a0 + a.la : a.na;  // output planes [a0, a1)
+  const int b0 = bt * a.tb;
+  const int nc = a.nc, na = a.na, nb = a.nb, P = a.plane;
+  const int tid = (int)threadIdx.x;
+  const int r = tid >> a.nc_log2;
+  const int c = tid & (nc - 1);
+  const int b = (b0 - 1 + r + nb) % nb;
+  const bool inner = r >= 1 && r <= a.tb && b0 + r - 1 < nb;
+  const T* src = static_cast<const T*>(a.src);
+  T* dst = static_cast<T*>(a.dst);
+
+  auto fetch = [&](int p, C (&v)[19]) {
+    Nbr n;
+    n.init((p + na) % na, b, c, na, nb, nc);
+#pragma unroll
+    for (int q = 0; q < 19; ++q)
+      v[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
+  };
+
+  // own column's step-1 values: Z of plane p - 1 (e_a = 0), P1 / P2 of planes
+  // p - 1 / p - 2 (e_a = +1); only the entries of those classes are used
+  C Z[19], P1[19], P2[19], nx[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) Z[q] = P1[q] = P2[q] = C(0);
+  fetch(a0 - 1, nx);
+  for (int p = a0 - 1; p <= a1; ++p) {
+    C d[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) d[q] = nx[q];
+    {
+      C drho, rho, ua, ub, uc;
+      moments<L, C>(d, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C>(d, a.inv_tau, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {  // the value a store and reload would give
+        const T s = St::template enc<L>(q, d[q]);
+        d[q] = St::template load<L>(&s, q);
+      }
+    }
+    const bool out = p > a0;
+    if (out) {
+#pragma unroll
+      for (int q = 3; q < 19; ++q)
+        x[q - 3][tid] = L::ea[q] < 0 ? d[q] : (L::ea[q] == 0 ? Z[q] : P2[q]);
+    }
+    __syncthreads();
+    if (p < a1) fetch(p + 1, nx);
+    if (out && inner) {
+      C e[19];
+      e[0] = Z[0];   // (p - 1, b, c)
+      e[1] = P2[1];  // (p - 2, b, c)
+      e[2] = d[2];   // (p, b, c)
+#pragma unroll
+      for (int q = 3; q < 19; ++q)
+        e[q] = x[q - 3][((r - L::eb[q]) << a.nc_log2) + ((c - L::ec[q]) & (nc - 1))];
+      C drho, rho, ua, ub, uc;
+      moments<L, C>(e, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C>(e, a.inv_tau, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      const int cell = ((p - 1) * nb + b) * nc + c;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) St::template store<L>(dst + q * P + cell, q, e[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      if (L::ea[q] > 0) {
+        P2[q] = P1[q];
+        P1[q] = d[q];
+      }
+      if (L::ea[q] == 0) Z[q] = d[q];
+    }
+  }
+}
+
+}  // namespace mlbm

@@ -1,0 +1,122 @@
+"""GPU: two pull steps per launch (k_pair) against single launches and the oracle.
+
+The two-step launch must give the same bits as the fused single-step
+launches it replaces (it runs the same per-cell arithmetic, with the
+intermediate state passed through the storage type's encoding exactly as a
+store and reload would), so every comparison with single launches is
+`torch.equal`. Parity with the float64 oracle is 1e-5 relative for float32
+storage, as for the single-step kernels.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import lbm_numpy as orc
+
+pytestmark = pytest.mark.gpu
+
+# rows of 64, 128 and 256 cells; partial b-tiles (nb not a multiple of the
+# rows per workgroup - 2), partial a-chunks, axes of 1-3 cells
+SHAPES = [(64, 80, 128), (33, 50, 64), (3, 2, 64), (1, 1, 64), (2, 1, 128), (130, 7, 256),
+          (5, 3, 256)]
+
+
+def _state(k, shape, seed):
+    rng = np.random.default_rng(seed)
+    rho = torch.from_numpy(1 + 0.01 * rng.standard_normal(shape)).float()
+    u = torch.from_numpy(0.03 * rng.standard_normal(shape + (3,))).float()
+    return k.equilibrium(rho, u)
+
+
+def _run(shape, storage, pair, nsteps, seed=5, force=(1e-5, -2e-5, 3e-5)):
+    from maddening_amd.lbm import LBMKernel
+    k = LBMKernel(shape, lattice="D3Q19", tau=0.7, storage=storage, scheme="pull", pair=pair)
+    f = _state(k, shape, seed)
+    out = k.step(f, nsteps, force=list(force))
+    torch.cuda.synchronize()
+    return k, [t.cpu() for t in out]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("storage", ["f32", "f32dev"])
+@pytest.mark.parametrize("nsteps", [2, 3, 6])
+def test_pair_launches_equal_single_launches_bitwise(shape, storage, nsteps):
+    print(f"pair vs single: {shape} {storage} nsteps={nsteps}", flush=True)
+    k, forced = _run(shape, storage, "force", nsteps)
+    launches, pairs = k.schedule(nsteps)
+    assert pairs == (nsteps - 1) // 2 and launches == 2 + pairs + (nsteps - 1) % 2
+    _, single = _run(shape, storage, "off", nsteps)
+    for a, b in zip(forced, single):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("shape", [(6, 9, 64), (4, 3, 128)])
+def test_pair_launches_match_oracle_f32(shape):
+    """Forced two-step launches against the float64 oracle after 7 steps."""
+    from maddening_amd.lbm import LBMKernel
+    rng = np.random.default_rng(21)
+    rho0 = 1 + 0.01 * rng.standard_normal(shape)
+    u0 = 0.03 * rng.standard_normal(shape + (3,))
+    e, w, _ = orc.lattice(19)
+    f0 = orc.equilibrium(rho0, u0, e, w)
+    F = (1e-5, 0.0, -2e-5)
+    k = LBMKernel(shape, lattice="D3Q19", tau=0.8, storage="f32", scheme="pull", pair="force")
+    assert k.schedule(7)[1] == 3
+    f = k.from_aos(torch.from_numpy(np.ascontiguousarray(f0)).to("cuda", torch.float64))
+    f, rho, u, p = k.step(f, 7, force=list(F))
+    torch.cuda.synchronize()
+    ref = orc.run(f0, 7, tau=0.8, force=F)
+    g = k.to_aos(f, torch.float64).cpu().numpy()
+    assert np.max(np.abs(g - ref["f"])) / np.max(np.abs(ref["f"])) < 1e-5
+    assert np.max(np.abs(u.cpu().numpy() - ref["velocity"])) / np.max(np.abs(ref["velocity"])) < 1e-5
+    assert np.max(np.abs(rho.cpu().numpy() - ref["density"])) < 1e-5
+
+
+@pytest.mark.parametrize("storage", ["f32", "f32dev"])
+def test_pull_phase_3_equals_two_phase_1(storage):
+    from maddening_amd import _lib
+    from maddening_amd.lbm import LBMKernel, _stream
+    shape = (9, 20, 128)
+    k = LBMKernel(shape, lattice="D3Q19", tau=0.9, storage=storage, pair="force")
+    f = _state(k, shape, 3)
+    d = k.desc(force=[2e-5, 0.0, 1e-5])
+    L = _lib.lib()
+    s = _stream(k.device)
+    mid, two, one = k.empty(), k.empty(), k.empty()
+
+    def ptr(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def phase(ph, src, dst):
+        _lib.check(L.lbm_pull_phase(s, ctypes.byref(d), ph, ptr(src), ptr(dst), None, None, None,
+                                    None, None, None, None), "lbm_pull_phase")
+
+    phase(1, f, mid)
+    phase(1, mid, two)
+    phase(3, f, one)
+    torch.cuda.synchronize()
+    assert torch.equal(one.cpu(), two.cpu())
+
+
+def test_full_size_auto_pairs_equal_single_launches():
+    """256^3 f32dev, the lattice the library picks two-step launches for by
+    itself: 11 steps (5 two-step launches and one single) equal 11 single
+    fused launches bit for bit."""
+    from maddening_amd.lbm import LBMKernel
+    shape = (256, 256, 256)
+    res = []
+    for pair in ("auto", "off"):
+        k = LBMKernel(shape, lattice="D3Q19", tau=0.8, storage="f32dev", pair=pair)
+        launches, pairs = k.schedule(11)
+        assert pairs == (5 if pair == "auto" else 0)
+        f = _state(k, shape, 9)
+        out, rho, u, _ = k.step(f, 11)
+        torch.cuda.synchronize()
+        res.append((out.view(torch.int32).sum(dtype=torch.int64).item(),
+                    out.view(torch.int32)[:, ::17, ::5, ::3].cpu(), rho.cpu(), u.cpu()))
+        del f, out
+    assert res[0][0] == res[1][0]
+    for a, b in zip(res[0][1:], res[1][1:]):
+        assert torch.equal(a, b)

@@ -1,0 +1,72 @@
+"""Which lbm_step calls run two steps per launch (LBM_FLAG_PAIR_*), checked on
+the CPU through lbm_step_schedule, which makes no HIP call."""
+import ctypes
+
+import pytest
+
+
+def _desc(dims, storage=4, bb=0, flags=0, faces=False):
+    from maddening_amd import _lib
+    d = _lib.LbmDesc()
+    d.q, d.storage, d.bb_mode, d.tau, d.flags = 19, storage, bb, 0.8, flags
+    for i, n in enumerate(dims):
+        d.dims[i] = n
+    if faces:
+        d.inlet_face, d.inlet_rho = 5, 1.0
+    return d
+
+
+def _schedule(d, nsteps, with_fields=0):
+    from maddening_amd import _lib
+    nl, npair = ctypes.c_int(-1), ctypes.c_int(-1)
+    _lib.check(_lib.lib().lbm_step_schedule(ctypes.byref(d), nsteps, with_fields,
+                                            ctypes.byref(nl), ctypes.byref(npair)),
+               "lbm_step_schedule")
+    return nl.value, npair.value
+
+
+@pytest.mark.parametrize("storage", [0, 4])  # f32, f32dev
+@pytest.mark.parametrize("nsteps", [2, 3, 1000, 1001])
+def test_auto_pairs_at_256_cubed(storage, nsteps):
+    nl, npair = _schedule(_desc((256, 256, 256), storage), nsteps)
+    assert npair == (nsteps - 1) // 2
+    assert nl == 2 + npair + (nsteps - 1) % 2
+
+
+def test_auto_keeps_single_launches_where_pairs_measured_slower():
+    from maddening_amd import _lib
+    # 16-bit and 64-bit storage, rows of 128 cells, too few workgroups,
+    # walls, pressure faces, force or tau fields, the push scheme, PAIR_OFF
+    cases = [_desc((256, 256, 256), storage=2), _desc((256, 256, 256), storage=3),
+             _desc((256, 256, 256), storage=1), _desc((128, 128, 128)),
+             _desc((16, 16, 256)), _desc((256, 256, 256), bb=1),
+             _desc((256, 256, 256), faces=True),
+             _desc((256, 256, 256), flags=_lib.LBM_FLAG_PAIR_OFF)]
+    for d in cases:
+        assert _schedule(d, 100) == (101, 0)
+    assert _schedule(_desc((256, 256, 256)), 100, with_fields=1) == (101, 0)
+    d = _desc((256, 256, 256))
+    d.scheme = _lib.LBM_SCHEME_PUSH
+    assert _schedule(d, 100) == (101, 0)
+
+
+def test_force_flag_reaches_every_supported_row_length():
+    from maddening_amd import _lib
+    F = _lib.LBM_FLAG_PAIR_FORCE
+    for dims in [(1, 1, 64), (33, 50, 64), (128, 128, 128), (3, 2, 256)]:
+        assert _schedule(_desc(dims, flags=F), 9) == (6, 4)
+    # not for rows the kernel does not cover, nor 16-bit storage
+    for d in [_desc((8, 8, 96), flags=F), _desc((8, 8, 512), flags=F),
+              _desc((8, 8, 64), storage=2, flags=F)]:
+        assert _schedule(d, 9) == (10, 0)
+
+
+def test_phase_3_rejected_where_pairs_do_not_apply():
+    from maddening_amd import _lib
+    L = _lib.lib()
+    d = _desc((8, 8, 64), storage=2, flags=_lib.LBM_FLAG_PAIR_FORCE)
+    rc = L.lbm_pull_phase(None, ctypes.byref(d), 3, ctypes.c_void_p(4096), ctypes.c_void_p(8192),
+                          None, None, None, None, None, None, None)
+    assert rc == -2 and b"two steps per launch" in L.lbm_last_error()
+    assert L.lbm_pull_phase(None, ctypes.byref(d), 4, ctypes.c_void_p(4096),
+                            ctypes.c_void_p(8192), None, None, None, None, None, None, None) == -1
