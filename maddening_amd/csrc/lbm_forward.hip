@@ -530,7 +530,6 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   pa->nc = g.nc;
   pa->plane = (int)g.plane;
   pa->nc_log2 = nc_log2;
-  pa->rows_log2 = 10 - nc_log2;
   pa->tb = tb;
   pa->btiles = (int)btiles;
   pa->la = la;

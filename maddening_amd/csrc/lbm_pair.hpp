@@ -43,8 +43,7 @@ struct PairArgs {
   void* dst;
   int na, nb, nc;  // nc in {64, 128, 256}
   int plane;       // na * nb * nc; 19 * plane < 2^31, so offsets fit 32 bits
-  int rows_log2;   // log2(1024 / nc): rows per workgroup
-  int nc_log2;
+  int nc_log2;     // log2(nc); the workgroup spans 1024 >> nc_log2 rows
   int tb;          // output rows per workgroup = 1024 / nc - 2
   int btiles;      // ceil(nb / tb)
   int la;          // planes per chunk
