@@ -588,14 +588,15 @@ def main():
     b_cell = 2 * 19 * STORAGE_BYTES[args.storage]
     rv = auto_row_vec(args.storage, n, args.row_vec) if kern.scheme != "push" else 0
     if pairs:
-        # dominant kernel: the two-step launch, timed on its own. Per launch it
-        # must read the state once and write it once (b_cell bytes per cell)
-        # while it performs two cell updates per cell.
+        # dominant kernel: the two-step launch, timed on its own; it performs
+        # two cell updates per cell and reads and writes the state once
         per_launch_s = time_pair_launch(kern, f0)
     else:
         per_launch_s = ev / launches
-    achieved = b_cell * cells / per_launch_s / 1e9
-    effective = mlups * 1e6 * b_cell / 1e9
+    updates = 2 if pairs else 1
+    # algorithmic bytes: b_cell per cell update, times the updates one launch makes
+    achieved = updates * b_cell * cells / per_launch_s / 1e9
+    traffic = read_traffic(args.storage, pair=bool(pairs))
     result = {
         "metric": "D3Q19 MLUPS & %HBM peak (fp32 vs fp16-dev storage); fwd+adjoint ms/step",
         "value": round(mlups, 1),
@@ -615,15 +616,18 @@ def main():
                    "two_step_launches": pairs, "wall_s_timed_call": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(args.storage, pair=bool(pairs)),
+                     "traffic": traffic,
                      "kernel": f"{fused_kernel_label(args.storage, rv, pairs)} "
-                               f"({b_cell} B/cell per launch algorithmic)",
+                               f"({b_cell} B algorithmic per cell update, "
+                               f"{updates} update(s) per cell per launch)",
                      "avg_launch_us": round(per_launch_s * 1e6, 2),
-                     "updates_per_launch": 2 if pairs else 1,
-                     # MLUPS x b_cell: the HBM rate a one-update-per-pass step
-                     # would need for this update rate
-                     "effective_GBs": round(effective, 1),
-                     "effective_frac": round(effective / HBM_PEAK_GBS, 4)},
+                     "updates_per_launch": updates,
+                     # HBM bytes the launch actually moves (rocprofv3
+                     # FETCH_SIZE + WRITE_SIZE) over its live duration
+                     "hbm_GBs": (round(traffic / per_launch_s / 1e9, 1)
+                                 if traffic else None),
+                     "hbm_frac": (round(traffic / per_launch_s / 1e9 / HBM_PEAK_GBS, 4)
+                                  if traffic else None)},
     }
     if pairs:
         # the same workload with one update per launch (LBM_FLAG_PAIR_OFF)
@@ -648,7 +652,7 @@ def main():
         ev2, _, launches2, pairs2, _ = time_kernel(k2, f2, args.steps, args.warmup)
         bc2 = 2 * 19 * STORAGE_BYTES[storage]
         extra[storage] = {"MLUPS": round(cells * args.steps / ev2 / 1e6, 1),
-                          "effective_GBs": round(bc2 * cells * args.steps / ev2 / 1e9, 1),
+                          "algorithmic_GBs": round(bc2 * cells * args.steps / ev2 / 1e9, 1),
                           "two_step_launches": pairs2,
                           "speedup_vs_headline": round(ev / ev2, 3), "_ev": ev2}
         if pairs2:
