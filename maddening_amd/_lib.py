@@ -110,6 +110,7 @@ _PROTOS = {
                                              ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_void_p),
                                              ctypes.POINTER(ctypes.c_longlong)]),
     "lbm_batch_plan_destroy": (ctypes.c_int, [_vp]),
+    "lbm_batch_plan_pairs": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbm_batch_run": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
     "lbm_batch_graph_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_void_p)]),

@@ -36,11 +36,15 @@ class LatticeBatch:
     ``specs``: sequence of dicts with keys ``lattice`` ("D3Q19"/"D2Q9"),
     ``shape``, ``tau``, optional ``force`` (D,), ``bb`` ("none",
     "reference", "halfway"), ``mask`` (uint8 array), ``lid_velocity``.
+    ``pairs``: periodic D3Q19 lattices with rows of at most 341 cells run
+    two steps per launch (same bits; ``lbm_batch_plan_pairs``). Off by
+    default: on the benchmark batch it measured slower.
     """
 
     def __init__(self, specs: Sequence[dict], *, cells_per_thread: int = 1, device="cuda",
-                 plane_align: int = 64, link_words: bool = True):
+                 plane_align: int = 64, link_words: bool = True, pairs: bool = False):
         self.device = torch.device(device)
+        self.pairs = bool(pairs)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
         if plane_align < 1:
@@ -109,6 +113,7 @@ class LatticeBatch:
         _lib.check(L.lbm_batch_plan_create(_stream(self.device), items, n, int(cells_per_thread),
                                            _ptr(plan), ctypes.byref(handle), ctypes.byref(tiles)),
                    "lbm_batch_plan_create")
+        _lib.check(L.lbm_batch_plan_pairs(handle, int(self.pairs)), "lbm_batch_plan_pairs")
         self._plan = plan
         self._handle = handle
         self.cells_per_thread = int(cells_per_thread)

@@ -12,6 +12,13 @@
 // f[(k + 1) % 2]; launch 0 collides, launches 1 .. n-1 stream and collide,
 // launch n streams (and writes moments where an entry asks for them).
 // A whole n-step run can be captured once into a HIP graph and replayed.
+//
+// Periodic D3Q19 lattices with rows of at most 341 cells form a class of
+// their own, which runs its fused steps two per launch (k_batch_pair, the
+// march of lbm_pair.hpp over a second tile table: one 1024-thread
+// workgroup per b-tile and a-chunk of each lattice). The number of such
+// launches is even, so every lattice's final state is in the buffer where
+// one launch per step leaves it, and the results are the same bits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,8 +30,15 @@
 #include "lbm_cell.hpp"
 #include "lbm_common.hpp"
 #include "lbm_host.hpp"
+#include "lbm_pair.hpp"
 
 namespace mlbm {
+
+// planes per a-chunk of a two-step workgroup in a batch (the 32 periodic
+// cubes of edge 32-160: 2685 workgroups; the time model rounds x (chunk + 2)
+// planes puts 16 and 32 within 2 % of each other, 64 13 % behind)
+constexpr int kBatchPairChunk = 32;
+constexpr int kPairXcdRun = 16;  // consecutive two-step tiles dealt to one XCD
 
 struct BatchEntry {
   int q, bb;
@@ -32,8 +46,9 @@ struct BatchEntry {
   long long plane;       // cells
   long long pitch;       // cells from one population plane to the next
   long long tile_begin;
+  long long pair_begin;  // first two-step tile (two-step classes only)
+  int tb, btiles, la;    // two-step tiling: output rows, b-tiles, a-chunk planes
   float inv_tau, Fa, Fb, Fc, lid_a, lid_b, lid_c;
-  int pad;
   void* buf[2];
   const uint8_t* mask;
   const uint32_t* links;  // halfway link words, or null (mask path)
@@ -101,12 +116,52 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// A lattice class: entries [e0, e1) with tiles [t0, t1), same (q, wall rule).
+// A lattice class: entries [e0, e1) with tiles [t0, t1), same (q, wall rule);
+// `pair`: periodic D3Q19 entries that also have two-step tiles [p0, p1).
 struct BatchGroup {
-  int q, bb;
+  int q, bb, pair;
   int e0, e1;
   long long t0, t1;
+  long long p0, p1;
 };
+
+// Two fused steps of every lattice of a two-step class: workgroup t of
+// [p0, p1) runs b-tile and a-chunk t - pair_begin of its entry.
+__global__ void __launch_bounds__(1024)
+    k_batch_pair(const void* plan, int parity, long long p0, int nblocks, int remap, int e0,
+                 int e1) {
+  __shared__ float x[16][1024];
+  const BatchEntry* items = entries(plan);
+  const long long t = p0 + xcd_remap(blockIdx.x, nblocks, remap);
+  int lo = e0, hi = e1 - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].pair_begin <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  // every field is the same across the workgroup: read it once into scalar
+  // registers, so the march keeps its vector registers (no spills)
+  auto u32 = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  auto uf = [&](float v) { return __int_as_float(u32(__float_as_int(v))); };
+  auto up = [&](void* v) {
+    const unsigned long long w = (unsigned long long)v;
+    const unsigned lo32 = (unsigned)u32((int)(w & 0xffffffffu));
+    const unsigned hi32 = (unsigned)u32((int)(w >> 32));
+    return (void*)(((unsigned long long)hi32 << 32) | lo32);
+  };
+  const BatchEntry& e = items[u32(lo)];
+  const int local = u32((int)(t - e.pair_begin));
+  PairArgs a;
+  a.src = up(e.buf[parity]);
+  a.dst = up(e.buf[parity ^ 1]);
+  a.na = u32(e.na); a.nb = u32(e.nb); a.nc = u32(e.nc);
+  a.pitch = u32((int)e.pitch);
+  a.tb = u32(e.tb); a.btiles = u32(e.btiles); a.la = u32(e.la);
+  a.nblocks = 0; a.remap = 0;
+  a.inv_tau = uf(e.inv_tau);
+  a.Fa = uf(e.Fa); a.Fb = uf(e.Fb); a.Fc = uf(e.Fc);
+  pair_march<float>(a, local % a.btiles, local / a.btiles, x);
+}
 
 template <class L, int BB>
 int group_launch(hipStream_t s, const void* dplan, const BatchGroup& gr, int phase, int parity,
@@ -147,13 +202,17 @@ struct lbm_batch_plan_t {
   const void* dplan;
   std::vector<mlbm::BatchGroup> groups;
   std::vector<mlbm::LinkJob> links;
+  int pairs = 0;  // run two-step classes two steps per launch (lbm_batch_plan_pairs)
 };
 
 namespace mlbm {
 
+// which: 0 every class, 1 classes stepped one step per launch, 2 two-step classes
 inline int batch_launch(hipStream_t s, const lbm_batch_plan_t* p, int phase, int parity,
-                        int grid) {
+                        int grid, int which = 0) {
   for (const BatchGroup& gr : p->groups) {
+    const bool two = p->pairs && gr.pair;
+    if ((which == 1 && two) || (which == 2 && !two)) continue;
     int rc;
     if (gr.q == 19) {
       if (gr.bb == BB_NONE) rc = group_launch<Lattice<19>, BB_NONE>(s, p->dplan, gr, phase, parity, grid);
@@ -182,7 +241,27 @@ inline int batch_run(hipStream_t s, const lbm_batch_plan_t* p, int nsteps, int g
     if (int rc = check_launch("k_links launch")) return rc;
   }
   int rc = batch_launch(s, p, 0, 0, grid);
-  for (int k = 1; k < nsteps && rc == LBM_OK; ++k) rc = batch_launch(s, p, 1, k & 1, grid);
+  bool two = false;
+  for (const BatchGroup& gr : p->groups) two = two || (p->pairs && gr.pair);
+  for (int k = 1; k < nsteps && rc == LBM_OK; ++k)
+    rc = batch_launch(s, p, 1, k & 1, grid, two ? 1 : 0);
+  if (two) {
+    // an even number of two-step launches keeps each lattice's parity: after
+    // them the state is back in buf[1], where step 1 + 2 * npair expects it
+    const int npair = 2 * ((nsteps - 1) / 4);
+    int par = 1;
+    for (int i = 0; i < npair && rc == LBM_OK; ++i, par ^= 1) {
+      for (const BatchGroup& gr : p->groups) {
+        if (!gr.pair) continue;
+        const long long n = gr.p1 - gr.p0;
+        hipLaunchKernelGGL(k_batch_pair, dim3((unsigned)n), dim3(1024), 0, s, p->dplan, par,
+                           gr.p0, (int)n, kPairXcdRun, gr.e0, gr.e1);
+        if ((rc = check_launch("k_batch_pair launch"))) break;
+      }
+    }
+    for (int k = 1 + 2 * npair; k < nsteps && rc == LBM_OK; ++k)
+      rc = batch_launch(s, p, 1, k & 1, grid, 2);
+  }
   if (rc == LBM_OK) rc = batch_launch(s, p, 2, nsteps & 1, grid);
   return rc;
 }
@@ -241,16 +320,24 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     return (items[i].bb_mode == LBM_BB_HALFWAY && items[i].links) ? int(BB_HALF_LINKS)
                                                                    : items[i].bb_mode;
   };
+  // periodic D3Q19 lattices the two-step march covers: rows of at most 341
+  // cells, 19 planes of `pitch` cells addressable with 32-bit offsets
+  auto pairable = [&](int i) {
+    const long long pitch = items[i].pitch ? items[i].pitch : geo[i].plane;
+    return items[i].q == 19 && items[i].bb_mode == LBM_BB_NONE && geo[i].nc <= 341 &&
+                   19LL * pitch < (1LL << 31)
+               ? 1
+               : 0;
+  };
+  auto cls = [&](int i) { return (items[i].q * 4 + kernel_bb(i)) * 2 + pairable(i); };
   std::vector<int> order(nitems);
   for (int i = 0; i < nitems; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    return items[x].q * 4 + kernel_bb(x) < items[y].q * 4 + kernel_bb(y);
-  });
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cls(x) < cls(y); });
   std::vector<char> host(lbm_batch_plan_size(nitems), 0);
   BatchEntry* out = reinterpret_cast<BatchEntry*>(host.data() + kHeaderBytes);
   lbm_batch_plan_t* hp = new lbm_batch_plan_t;
   hp->dplan = device_plan;
-  long long tiles = 0;
+  long long tiles = 0, ptiles = 0;
   for (int j = 0; j < nitems; ++j) {
     const int i = order[j];
     const lbm_batch_item& it = items[i];
@@ -263,6 +350,16 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     e.plane = g.plane;
     e.pitch = it.pitch ? it.pitch : g.plane;
     e.tile_begin = tiles;
+    const int two = pairable(i);
+    e.pair_begin = ptiles;
+    e.tb = e.btiles = e.la = 0;
+    long long np = 0;
+    if (two) {
+      e.tb = 1024 / g.nc - 2;
+      e.btiles = (g.nb + e.tb - 1) / e.tb;
+      e.la = kBatchPairChunk;
+      np = (long long)e.btiles * ((g.na + e.la - 1) / e.la);
+    }
     e.inv_tau = float(1.0 / it.tau);
     if (it.q == 19) {
       e.Fa = float(it.force[0]); e.Fb = float(it.force[1]); e.Fc = float(it.force[2]);
@@ -284,11 +381,20 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     e.u = it.u;
     e.p = it.p;
     const long long nt = (g.plane + tile_cells - 1) / tile_cells;
-    if (hp->groups.empty() || hp->groups.back().q != it.q || hp->groups.back().bb != e.bb)
-      hp->groups.push_back(BatchGroup{it.q, e.bb, j, j, tiles, tiles});
+    if (hp->groups.empty() || hp->groups.back().q != it.q || hp->groups.back().bb != e.bb ||
+        hp->groups.back().pair != two)
+      hp->groups.push_back(BatchGroup{it.q, e.bb, two, j, j, tiles, tiles, ptiles, ptiles});
     hp->groups.back().e1 = j + 1;
     hp->groups.back().t1 = tiles + nt;
+    hp->groups.back().p1 = ptiles + np;
     tiles += nt;
+    ptiles += np;
+  }
+  for (const BatchGroup& gr : hp->groups) {
+    if (gr.p1 - gr.p0 >= (1LL << 31)) {
+      delete hp;
+      return fail(LBM_ERR_UNSUPPORTED, "too many two-step workgroups in one lattice class");
+    }
   }
   BatchHeader* hdr = reinterpret_cast<BatchHeader*>(host.data());
   hdr->total_tiles = tiles;
@@ -303,6 +409,13 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
     return fail(LBM_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(err));
   }
   *plan = hp;
+  return LBM_OK;
+}
+
+int lbm_batch_plan_pairs(lbm_batch_plan_t* plan, int on) {
+  g_last_error.clear();
+  if (!plan) return fail(LBM_ERR_INVALID, "plan is NULL");
+  plan->pairs = on ? 1 : 0;
   return LBM_OK;
 }
 

@@ -500,24 +500,27 @@ constexpr bool pair_storage() {
 
 // Plan of the two-step launch for lattice g, or false where it does not
 // apply: D3Q19, periodic, 32-bit storage, uniform force and tau (`fields`:
-// the call passes a force or tau field), no pressure faces, rows of 64, 128
-// or 256 cells, and 19 planes addressable with 32-bit offsets.
+// the call passes a force or tau field), no pressure faces, rows of at most
+// 341 cells (three rows per 1024-thread workgroup), and 19 planes
+// addressable with 32-bit offsets.
 // Without LBM_FLAG_PAIR_FORCE it is used only where it was measured faster
 // than two fused launches: rows of 256 cells, chunks of at least 64 planes
 // and at least 256 workgroups, one per CU (256^3 f32dev: 626 us per launch
-// against 910 us for two k_cell launches, bit-identical results). With rows
-// of 128 cells and 22-43 workgroups it measured 2-4x slower.
+// against 910 us for two k_cell launches, bit-identical results). Other
+// shapes won or lost depending on how many workgroups their chunks leave
+// (f32, per two steps: 256x256x160 410 against 583 us, 128^3 108 against
+// 147, 160^3 209 against 240, 200x200x100 221 against 224, 320x320x192 1041
+// against 1110; rows of 128 with 22-43 workgroups 2-4x slower).
 inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
   if (d->flags & LBM_FLAG_PAIR_OFF) return false;
   const bool force = (d->flags & LBM_FLAG_PAIR_FORCE) != 0;
   if (d->q != 19 || d->bb_mode != LBM_BB_NONE || fields) return false;
   if (d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
   if (d->inlet_face > 0 || d->outlet_face > 0) return false;
-  if (g.nc != 64 && g.nc != 128 && g.nc != 256) return false;
+  if (g.nc > 341) return false;
   if (19LL * g.plane >= (1LL << 31)) return false;
   if (!force && g.nc != 256) return false;
-  const int nc_log2 = g.nc == 64 ? 6 : (g.nc == 128 ? 7 : 8);
-  const int tb = (1024 >> nc_log2) - 2;
+  const int tb = 1024 / g.nc - 2;
   const long long btiles = (g.nb + tb - 1) / tb;
   int la = 128;
   auto blocks = [&](int l) { return btiles * ((g.na + l - 1) / l); };
@@ -528,8 +531,7 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   pa->na = g.na;
   pa->nb = g.nb;
   pa->nc = g.nc;
-  pa->plane = (int)g.plane;
-  pa->nc_log2 = nc_log2;
+  pa->pitch = (int)g.plane;
   pa->tb = tb;
   pa->btiles = (int)btiles;
   pa->la = la;

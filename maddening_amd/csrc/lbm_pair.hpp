@@ -5,8 +5,9 @@
 // writes g_{n+1}(x) = C(f_{n+1}(x)): 152 B of HBM per cell update in float32.
 // k_pair computes g_{n+2} from g_n in one pass, so the state crosses HBM once
 // per two updates. A 1024-thread workgroup owns R = 1024 / nc whole rows of
-// the contiguous axis c (nc in {64, 128, 256}), rows b0 - 1 .. b0 + R - 2 of
-// axis b, and marches along axis a over a chunk of planes:
+// the contiguous axis c (nc <= 341, so R >= 3; threads past R * nc only join
+// the barriers), rows b0 - 1 .. b0 + R - 2 of axis b, and marches along axis
+// a over a chunk of planes:
 //
 //   plane p:  step 1 at (p, row, c) for all R rows: gather g_n from HBM
 //             (the next plane's gather is issued before step 2, so its
@@ -41,9 +42,9 @@ namespace mlbm {
 struct PairArgs {
   const void* src;
   void* dst;
-  int na, nb, nc;  // nc in {64, 128, 256}
-  int plane;       // na * nb * nc; 19 * plane < 2^31, so offsets fit 32 bits
-  int nc_log2;     // log2(nc); the workgroup spans 1024 >> nc_log2 rows
+  int na, nb, nc;  // nc <= 341
+  int pitch;       // cells from one population plane to the next (>= na * nb * nc);
+                   // 19 * pitch < 2^31, so offsets fit 32 bits
   int tb;          // output rows per workgroup = 1024 / nc - 2
   int btiles;      // ceil(nb / tb)
   int la;          // planes per chunk
@@ -52,22 +53,23 @@ struct PairArgs {
   float inv_tau, Fa, Fb, Fc;
 };
 
+// The march of one workgroup over b-tile bt and a-chunk `chunk` of lattice a;
+// x is the workgroup's 64 KB exchange buffer.
 template <class T>
-__global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
+__device__ __forceinline__ void pair_march(const PairArgs& a, int bt, int chunk,
+                                           float (&x)[16][1024]) {
   using L = Lattice<19>;
   using St = Storage<T>;
   using C = float;
   static_assert(sizeof(typename Storage<T>::C) == 4, "float compute only");
-  __shared__ float x[16][1024];
-  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
-  const int bt = lb % a.btiles;
-  const int a0 = (lb / a.btiles) * a.la;
+  const int a0 = chunk * a.la;
   const int a1 = a0 + a.la < a.na ? a0 + a.la : a.na;  // output planes [a0, a1)
   const int b0 = bt * a.tb;
-  const int nc = a.nc, na = a.na, nb = a.nb, P = a.plane;
+  const int nc = a.nc, na = a.na, nb = a.nb, P = a.pitch;
   const int tid = (int)threadIdx.x;
-  const int r = tid >> a.nc_log2;
-  const int c = tid & (nc - 1);
+  const int r = tid / nc;
+  const int c = tid - r * nc;
+  const bool active = r < a.tb + 2;
   const int b = (b0 - 1 + r + nb) % nb;
   const bool inner = r >= 1 && r <= a.tb && b0 + r - 1 < nb;
   const T* src = static_cast<const T*>(a.src);
@@ -86,7 +88,7 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   C Z[19], P1[19], P2[19], nx[19];
 #pragma unroll
   for (int q = 0; q < 19; ++q) Z[q] = P1[q] = P2[q] = C(0);
-  fetch(a0 - 1, nx);
+  if (active) fetch(a0 - 1, nx);
   for (int p = a0 - 1; p <= a1; ++p) {
     C d[19];
 #pragma unroll
@@ -102,21 +104,24 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
       }
     }
     const bool out = p > a0;
-    if (out) {
+    if (out && active) {
 #pragma unroll
       for (int q = 3; q < 19; ++q)
         x[q - 3][tid] = L::ea[q] < 0 ? d[q] : (L::ea[q] == 0 ? Z[q] : P2[q]);
     }
     __syncthreads();
-    if (p < a1) fetch(p + 1, nx);
+    if (p < a1 && active) fetch(p + 1, nx);
     if (out && inner) {
       C e[19];
       e[0] = Z[0];   // (p - 1, b, c)
       e[1] = P2[1];  // (p - 2, b, c)
       e[2] = d[2];   // (p, b, c)
 #pragma unroll
-      for (int q = 3; q < 19; ++q)
-        e[q] = x[q - 3][((r - L::eb[q]) << a.nc_log2) + ((c - L::ec[q]) & (nc - 1))];
+      for (int q = 3; q < 19; ++q) {
+        int cc = c - L::ec[q];  // periodic along c, inside the workgroup's rows
+        cc = cc < 0 ? cc + nc : (cc >= nc ? cc - nc : cc);
+        e[q] = x[q - 3][(r - L::eb[q]) * nc + cc];
+      }
       C drho, rho, ua, ub, uc;
       moments<L, C>(e, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
       collide_for<L, T, C>(e, a.inv_tau, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
@@ -134,6 +139,13 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
       if (L::ea[q] == 0) Z[q] = d[q];
     }
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
+  __shared__ float x[16][1024];
+  const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
+  pair_march<T>(a, lb % a.btiles, lb / a.btiles, x);
 }
 
 }  // namespace mlbm

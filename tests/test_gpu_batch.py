@@ -119,3 +119,46 @@ def test_batch_link_words_follow_mask_updates():
     batch.run(nsteps)
     torch.cuda.synchronize()
     assert torch.equal(batch.states(nsteps)[i], ref[i])
+
+
+def pair_specs():
+    """Periodic D3Q19 lattices of several row lengths (two-step class) beside
+    lattices that stay one step per launch."""
+    return [
+        {"lattice": "D3Q19", "shape": (40, 33, 100), "tau": 0.8, "bb": "none",
+         "force": (1e-5, 0.0, -1e-5)},
+        {"lattice": "D3Q19", "shape": (9, 70, 160), "tau": 0.6, "bb": "none"},
+        {"lattice": "D3Q19", "shape": (3, 2, 341), "tau": 1.4, "bb": "none"},
+        {"lattice": "D3Q19", "shape": (1, 5, 7), "tau": 0.7, "bb": "none"},
+        {"lattice": "D3Q19", "shape": (4, 4, 400), "tau": 0.9, "bb": "none"},   # rows > 341
+        {"lattice": "D2Q9", "shape": (64, 48), "tau": 0.9, "bb": "none"},
+        {"lattice": "D3Q19", "shape": (8, 9, 20), "tau": 0.55, "bb": "reference",
+         "mask": (np.random.default_rng(5).random((8, 9, 20)) < 0.2).astype(np.uint8)},
+    ]
+
+
+@pytest.mark.parametrize("nsteps", [2, 3, 5, 6, 9, 14])
+def test_batch_two_step_launches_equal_single_launches(nsteps):
+    """The two-step class (an even number of two-step launches, the rest one
+    step per launch) equals one launch per step and lbm_step per lattice, bit
+    for bit, eagerly and through a replayed graph."""
+    from maddening_amd.batch import LatticeBatch
+    print(f"batch two-step vs single: nsteps={nsteps}", flush=True)
+    on = LatticeBatch(pair_specs(), cells_per_thread=2, pairs=True)
+    off = LatticeBatch(pair_specs(), cells_per_thread=2, pairs=False)
+    f0s = initial_states(on, seed=11)
+    ref = reference_results(on, f0s, nsteps)
+    for b in (on, off):
+        b.set_states(f0s)
+        b.run(nsteps)
+    torch.cuda.synchronize()
+    for i, (a, c, r) in enumerate(zip(on.states(nsteps), off.states(nsteps), ref)):
+        assert torch.equal(a, c), (nsteps, i)
+        assert torch.equal(a, r), (nsteps, i)
+    h = on.graph(nsteps)
+    for _ in range(2):
+        on.set_states(f0s)
+        on.launch_graph(h)
+        torch.cuda.synchronize()
+        for i, (a, r) in enumerate(zip(on.states(nsteps), ref)):
+            assert torch.equal(a, r), ("graph", nsteps, i)

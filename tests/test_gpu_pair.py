@@ -17,10 +17,11 @@ from oracle import lbm_numpy as orc
 
 pytestmark = pytest.mark.gpu
 
-# rows of 64, 128 and 256 cells; partial b-tiles (nb not a multiple of the
-# rows per workgroup - 2), partial a-chunks, axes of 1-3 cells
+# rows of 1 to 341 cells (1024 / nc rows per workgroup, at least three);
+# partial b-tiles (nb not a multiple of the rows per workgroup - 2), partial
+# a-chunks, axes of 1-3 cells
 SHAPES = [(64, 80, 128), (33, 50, 64), (3, 2, 64), (1, 1, 64), (2, 1, 128), (130, 7, 256),
-          (5, 3, 256)]
+          (5, 3, 256), (7, 9, 100), (4, 5, 341), (2, 2, 3), (1, 1, 1), (40, 37, 160)]
 
 
 def _state(k, shape, seed):

@@ -50,13 +50,15 @@ def test_auto_keeps_single_launches_where_pairs_measured_slower():
     assert _schedule(d, 100) == (101, 0)
 
 
-def test_force_flag_reaches_every_supported_row_length():
+def test_force_flag_reaches_rows_of_up_to_341_cells():
     from maddening_amd import _lib
     F = _lib.LBM_FLAG_PAIR_FORCE
-    for dims in [(1, 1, 64), (33, 50, 64), (128, 128, 128), (3, 2, 256)]:
+    for dims in [(1, 1, 64), (33, 50, 64), (128, 128, 128), (3, 2, 256), (8, 8, 96),
+                 (2, 3, 341), (1, 1, 1)]:
         assert _schedule(_desc(dims, flags=F), 9) == (6, 4)
-    # not for rows the kernel does not cover, nor 16-bit storage
-    for d in [_desc((8, 8, 96), flags=F), _desc((8, 8, 512), flags=F),
+    # not for rows of more than 341 cells (fewer than three per workgroup),
+    # nor 16-bit storage
+    for d in [_desc((8, 8, 342), flags=F), _desc((8, 8, 512), flags=F),
               _desc((8, 8, 64), storage=2, flags=F)]:
         assert _schedule(d, 9) == (10, 0)
 
