@@ -9,9 +9,11 @@ of the final state included.
 
 Prints one JSON line. `value` is MLUPS (million lattice-cell updates per
 second). `roofline` prices the dominant kernel at the algorithmic
-2 * Q * 4 = 152 bytes per cell and launch against 8 TB/s HBM3E;
-`cpu_baseline` times the float32 NumPy restatement of the reference step on
-this host (the oracle, used as a port of the reference's CPU path).
+2 * Q * 4 = 152 bytes per cell update, times the updates one launch makes
+(two for the two-step launch), against 8 TB/s HBM3E, with the launch's
+measured HBM traffic and rate beside it; `cpu_baseline` times the C
+restatement of the reference step (oracle/lbm_ref.c, OpenMP) on this
+host's cores, with the float32 NumPy restatement on one core beside it.
 """
 from __future__ import annotations
 
@@ -442,10 +444,27 @@ def batch_leg(dev, nsteps=200, seed=0):
     t_graph = e0.elapsed_time(e1) / 1e3
     t_loop = e1b.elapsed_time(e2) / 1e3
     batch.close()
+    # the same graph with the periodic D3Q19 lattices two steps per launch
+    # (lbm_batch_plan_pairs; bit-identical)
+    batch2 = LatticeBatch(specs, device=dev, pairs=True)
+    batch2.plan(tune["cells_per_thread"])
+    h2 = batch2.graph(nsteps, tune["grid"])
+    batch2.set_states(f0s)
+    batch2.launch_graph(h2)  # warm
+    torch.cuda.synchronize()
+    batch2.set_states(f0s)
+    e3, e4 = (torch.cuda.Event(enable_timing=True) for _ in range(2))
+    e3.record()
+    batch2.launch_graph(h2)
+    e4.record()
+    torch.cuda.synchronize()
+    t_pair = e3.elapsed_time(e4) / 1e3
+    batch2.close()
     return {"lattices": len(specs), "cells": batch.cells, "steps": nsteps,
             "graph_MLUPS": round(batch.cells * nsteps / t_graph / 1e6, 1),
             "per_lattice_MLUPS": round(batch.cells * nsteps / t_loop / 1e6, 1),
             "speedup": round(t_loop / t_graph, 3),
+            "two_step_graph_MLUPS": round(batch.cells * nsteps / t_pair / 1e6, 1),
             "tuned": {"cells_per_thread": tune["cells_per_thread"], "grid": tune["grid"]}}
 
 

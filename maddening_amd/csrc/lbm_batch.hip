@@ -15,7 +15,7 @@
 //
 // Periodic D3Q19 lattices with rows of at most 341 cells form a class of
 // their own, which runs its fused steps two per launch (k_batch_pair, the
-// march of lbm_pair.hpp over a second tile table: one 1024-thread
+// march of lbm_pair.hpp (its LDS-ring form) over a second tile table: one 1024-thread
 // workgroup per b-tile and a-chunk of each lattice). The number of such
 // launches is even, so every lattice's final state is in the buffer where
 // one launch per step leaves it, and the results are the same bits.
@@ -130,7 +130,7 @@ struct BatchGroup {
 __global__ void __launch_bounds__(1024)
     k_batch_pair(const void* plan, int parity, long long p0, int nblocks, int remap, int e0,
                  int e1) {
-  __shared__ float x[16][1024];
+  __shared__ PairRing x;
   const BatchEntry* items = entries(plan);
   const long long t = p0 + xcd_remap(blockIdx.x, nblocks, remap);
   int lo = e0, hi = e1 - 1;
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(1024)
   a.nblocks = 0; a.remap = 0;
   a.inv_tau = uf(e.inv_tau);
   a.Fa = uf(e.Fa); a.Fb = uf(e.Fb); a.Fc = uf(e.Fc);
-  pair_march<float>(a, local % a.btiles, local / a.btiles, x);
+  pair_march_ring<float>(a, local % a.btiles, local / a.btiles, x);
 }
 
 template <class L, int BB>

@@ -141,6 +141,99 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int bt, int chunk,
   }
 }
 
+// The same two steps with every step-1 value held in LDS rings instead of
+// registers (152 KB: e_a = -1 values for one plane, e_a = 0 for two, e_a = +1
+// for three), for callers that need registers of their own: the batch
+// kernel reads each workgroup's lattice from a table, and with the register
+// carry above it spilled inside the plane loop, where every reload waits for
+// the prefetched gather (5.06 ms per launch on the benchmark batch). Same
+// per-cell arithmetic, same bits.
+struct PairRing {
+  float v[38][1024];  // rows: M 0-4 | Z 5 + 9 * (t & 1) | P 23 + 5 * (t % 3)
+};
+
+__host__ __device__ constexpr int ring_class_slot(int q) {
+  // M: 2, 8, 10, 12, 14; Z: 0, 3, 4, 5, 6, 15, 16, 17, 18; P: 1, 7, 9, 11, 13
+  return q == 2 ? 0 : q == 8 ? 1 : q == 10 ? 2 : q == 12 ? 3 : q == 14 ? 4 :
+         q == 0 ? 0 : q == 3 ? 1 : q == 4 ? 2 : q == 5 ? 3 : q == 6 ? 4 : q == 15 ? 5 :
+         q == 16 ? 6 : q == 17 ? 7 : q == 18 ? 8 :
+         q == 1 ? 0 : q == 7 ? 1 : q == 9 ? 2 : q == 11 ? 3 : 4;
+}
+
+// ring row of direction q written by step-1 iteration t
+template <class L>
+__device__ __forceinline__ int ring_row(int q, int t) {
+  return L::ea[q] < 0 ? ring_class_slot(q)
+                      : (L::ea[q] == 0 ? 5 + (t & 1) * 9 + ring_class_slot(q)
+                                       : 23 + (t % 3) * 5 + ring_class_slot(q));
+}
+
+template <class T>
+__device__ __forceinline__ void pair_march_ring(const PairArgs& a, int bt, int chunk,
+                                                PairRing& x) {
+  using L = Lattice<19>;
+  using St = Storage<T>;
+  using C = float;
+  static_assert(sizeof(typename Storage<T>::C) == 4, "float compute only");
+  const int a0 = chunk * a.la;
+  const int a1 = a0 + a.la < a.na ? a0 + a.la : a.na;  // output planes [a0, a1)
+  const int b0 = bt * a.tb;
+  const int nc = a.nc, na = a.na, nb = a.nb, P = a.pitch;
+  const int tid = (int)threadIdx.x;
+  const int r = tid / nc;
+  const int c = tid - r * nc;
+  const bool active = r < a.tb + 2;
+  const int b = (b0 - 1 + r + nb) % nb;
+  const bool inner = r >= 1 && r <= a.tb && b0 + r - 1 < nb;
+  const T* src = static_cast<const T*>(a.src);
+  T* dst = static_cast<T*>(a.dst);
+  auto fetch = [&](int p, C (&v)[19]) {
+    Nbr n;
+    n.init((p + na) % na, b, c, na, nb, nc);
+#pragma unroll
+    for (int q = 0; q < 19; ++q)
+      v[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
+  };
+  C nx[19];
+  if (active) fetch(a0 - 1, nx);
+  for (int p = a0 - 1, t = 0; p <= a1; ++p, ++t) {
+    if (active) {
+      C d[19];
+#pragma unroll
+      for (int q = 0; q < 19; ++q) d[q] = nx[q];
+      if (p < a1) fetch(p + 1, nx);
+      C drho, rho, ua, ub, uc;
+      moments<L, C>(d, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C>(d, a.inv_tau, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {  // the value a store and reload would give
+        const T s = St::template enc<L>(q, d[q]);
+        x.v[ring_row<L>(q, t)][tid] = St::template load<L>(&s, q);
+      }
+    }
+    __syncthreads();
+    if (p > a0 && inner) {
+      // step 2 at (p - 1, b, c): e_a = -1 from plane p (iteration t), 0 from
+      // p - 1 (t - 1: (t - 1) & 1 == (t + 1) & 1), +1 from p - 2 (t - 2:
+      // (t - 2) % 3 == (t + 1) % 3)
+      C e[19];
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        int cc = c - L::ec[q];  // periodic along c, inside the workgroup's rows
+        cc = cc < 0 ? cc + nc : (cc >= nc ? cc - nc : cc);
+        e[q] = x.v[ring_row<L>(q, L::ea[q] < 0 ? t : t + 1)][(r - L::eb[q]) * nc + cc];
+      }
+      C drho, rho, ua, ub, uc;
+      moments<L, C>(e, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C>(e, a.inv_tau, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      const int cell = ((p - 1) * nb + b) * nc + c;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) St::template store<L>(dst + q * P + cell, q, e[q]);
+    }
+    __syncthreads();
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   __shared__ float x[16][1024];
