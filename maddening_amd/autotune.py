@@ -60,7 +60,7 @@ def _time(fn, reps: int) -> float:
     return best
 
 
-_KNOBS = ("block_x", "row_vec", "scheme", "xcd_remap", "xcd_group")
+_KNOBS = ("block_x", "row_vec", "scheme", "xcd_remap", "xcd_group", "pair")
 
 
 def _apply(kern: LBMKernel, cand: dict) -> None:
@@ -71,20 +71,21 @@ def _apply(kern: LBMKernel, cand: dict) -> None:
 def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool = True) -> dict:
     """Pick the launch shape for ``kern``'s lattice: the library's own choice,
     row blocks of each width (pull or push), flattened 256-cell tiles, the
-    row-per-wave kernels at each legal vector width, then XCD run lengths for
-    the best of those. Applies and returns the winner with every candidate's
-    ms per step."""
+    row-per-wave kernels at each legal vector width, two steps per launch
+    where the lattice allows it (D3Q19, periodic, 32-bit storage, rows of at
+    most 341 cells), then XCD run lengths for the best of those. Applies and
+    returns the winner with every candidate's ms per step."""
     dev = torch.cuda.get_device_name(kern.device)
-    key = f"kernel2|{dev}|{kern.lat.name}|{kern.grid_shape}|{kern.storage}|{kern.bb}"
+    key = f"kernel3|{dev}|{kern.lat.name}|{kern.grid_shape}|{kern.storage}|{kern.bb}"
     if use_cache:
         hit = _load().get(key)
         if hit:
             _apply(kern, hit)
             return hit
 
-    def cand(block_x=0, row_vec=0, scheme="auto", xcd_remap=False, xcd_group=0):
+    def cand(block_x=0, row_vec=0, scheme="auto", xcd_remap=False, xcd_group=0, pair="auto"):
         return {"block_x": block_x, "row_vec": row_vec, "scheme": scheme,
-                "xcd_remap": xcd_remap, "xcd_group": xcd_group}
+                "xcd_remap": xcd_remap, "xcd_group": xcd_group, "pair": pair}
 
     shapes = [cand()]
     shapes += [cand(bx, -1, sc) for bx in (64, 128, 256) for sc in ("pull", "push")]
@@ -92,6 +93,8 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
     if kern.bb == "none":
         shapes += [cand(0, v, "pull") for v in (1, 2, 4)
                    if kern.grid_shape[-1] % (64 * v) == 0 and not (v == 4 and kern.storage == "f64")]
+        if kern.Q == 19 and kern.storage in ("f32", "f32dev") and kern.grid_shape[-1] <= 341:
+            shapes.append(cand(0, -1, "pull", pair="force"))
     f = kern.equilibrium()
     out = kern.empty()
     mask = torch.zeros(kern.grid_shape, dtype=torch.uint8, device=kern.device) \

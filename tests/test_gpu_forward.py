@@ -485,8 +485,9 @@ def test_tune_kernel_picks_a_candidate_and_keeps_results(tmp_path, monkeypatch):
     rec = autotune.tune_kernel(k, nsteps=3, reps=1)
     assert len(rec["candidates"]) >= 12
     assert rec["ms_per_step"] == min(c["ms_per_step"] for c in rec["candidates"])
-    assert (k.block_x, k.row_vec, k.scheme, k.xcd_remap, k.xcd_group) == \
-        (rec["block_x"], rec["row_vec"], rec["scheme"], rec["xcd_remap"], rec["xcd_group"])
+    assert (k.block_x, k.row_vec, k.scheme, k.xcd_remap, k.xcd_group, k.pair) == \
+        (rec["block_x"], rec["row_vec"], rec["scheme"], rec["xcd_remap"], rec["xcd_group"],
+         rec["pair"])
     assert autotune.tune_kernel(kernel(shape, "D3Q19", 0.8, bb="reference"))["ms_per_step"] == \
         rec["ms_per_step"]  # served from the cache
     rng = np.random.default_rng(0)
@@ -496,3 +497,19 @@ def test_tune_kernel_picks_a_candidate_and_keeps_results(tmp_path, monkeypatch):
     k.scheme = "pull"
     got = k.step(f0, 6, mask=mask)[0]
     assert torch.equal(got, ref)
+
+
+def test_tune_kernel_times_two_step_launches(tmp_path, monkeypatch):
+    """For a periodic D3Q19 float32 lattice the tuner also times two steps per
+    launch; whichever wins, the tuned kernel gives the default kernel's bits."""
+    from maddening_amd import autotune
+    monkeypatch.setattr(autotune, "_CACHE", str(tmp_path / "tune.json"))
+    shape = (24, 30, 100)
+    k = kernel(shape, "D3Q19", 0.8, storage="f32dev")
+    rec = autotune.tune_kernel(k, nsteps=4, reps=1)
+    assert any(c["pair"] == "force" for c in rec["candidates"])
+    assert k.pair == rec["pair"]
+    f0 = kernel(shape, "D3Q19", 0.8, storage="f32dev").equilibrium()
+    ref = kernel(shape, "D3Q19", 0.8, storage="f32dev", scheme="pull", pair="off").step(f0, 7)[0]
+    k.scheme = "pull"
+    assert torch.equal(k.step(f0, 7)[0], ref)
