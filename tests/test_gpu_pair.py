@@ -121,3 +121,33 @@ def test_full_size_auto_pairs_equal_single_launches():
     assert res[0][0] == res[1][0]
     for a, b in zip(res[0][1:], res[1][1:]):
         assert torch.equal(a, b)
+
+
+def test_pair_launches_random_shapes_bitwise():
+    """Seeded random extents (rows of 1-341 cells, other axes 1-40), step
+    counts 2-9, both 32-bit storages, random uniform force and tau: forced
+    two-step launches equal one launch per step bit for bit."""
+    from maddening_amd.lbm import LBMKernel
+    rng = np.random.default_rng(2024)
+    for case in range(24):
+        nc = int(rng.integers(1, 342))
+        na = int(rng.integers(1, 41))
+        nb = int(rng.integers(1, 41))
+        while na * nb * nc > 200_000:
+            na = max(1, na // 2)
+            nb = max(1, nb // 2)
+        shape = (na, nb, nc)
+        nsteps = int(rng.integers(2, 10))
+        storage = ("f32", "f32dev")[case % 2]
+        tau = float(rng.uniform(0.52, 1.8))
+        force = [float(v) for v in rng.uniform(-3e-5, 3e-5, 3)]
+        print(f"random case {case}: {shape} {storage} nsteps={nsteps} tau={tau:.3f}", flush=True)
+        outs = []
+        for pair in ("force", "off"):
+            k = LBMKernel(shape, lattice="D3Q19", tau=tau, storage=storage, scheme="pull",
+                          pair=pair)
+            f = _state(k, shape, case)
+            outs.append([t.cpu() for t in k.step(f, nsteps, force=force)])
+        torch.cuda.synchronize()
+        for a, b in zip(*outs):
+            assert torch.equal(a, b), (case, shape, storage, nsteps)
