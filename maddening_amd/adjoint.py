@@ -8,7 +8,10 @@ the last to the first. States the backward needs are recomputed from
 checkpoints: with ``checkpoint="sqrt"`` every K = ceil(sqrt(N))-th state is
 kept during the forward pass and each K-step segment is recomputed once
 (N/K + K states resident, about N extra launches); ``checkpoint="all"``
-keeps every state (N states resident) and recomputes nothing; an int sets K.
+keeps every state (N states resident) and recomputes nothing; an int sets K;
+``checkpoint="auto"`` keeps every state when N states fit in a fraction of
+the device memory that is free (config 4's 500 steps of 128x128x256 float32
+take 159 GB, which fits one MI355X's 288 GB), else uses sqrt.
 The forward pass itself is exactly lbm_step's N + 1 launches.
 
 Gradients flow to f0, to a uniform body force (tensor of shape (D,)) or a
@@ -24,6 +27,22 @@ import torch
 
 from . import _lib
 from .lbm import LBMKernel, _ptr, _stream
+
+
+# share of the free device memory "auto" lets the N states take
+AUTO_MEMORY_SHARE = 0.6
+
+
+def resolve_checkpoint(state_bytes: int, nsteps: int, checkpoint: Union[str, int],
+                       free_bytes: Optional[int] = None, device=None) -> Union[str, int]:
+    """``checkpoint`` with "auto" replaced by "all" when nsteps states of
+    ``state_bytes`` fit in AUTO_MEMORY_SHARE of ``free_bytes`` (default: the
+    device's free memory now), else by "sqrt"."""
+    if checkpoint != "auto":
+        return checkpoint
+    if free_bytes is None:
+        free_bytes = torch.cuda.mem_get_info(device)[0]
+    return "all" if nsteps * state_bytes <= AUTO_MEMORY_SHARE * free_bytes else "sqrt"
 
 
 def segment_length(nsteps: int, checkpoint: Union[str, int]) -> int:
@@ -241,6 +260,8 @@ def rollout(kern: LBMKernel, f0: torch.Tensor, nsteps: int, *,
         raise ValueError("nsteps must be >= 1")
     if force is not None and not torch.is_tensor(force):
         force = torch.as_tensor(force, dtype=torch.float64, device=kern.device)
+    state_bytes = kern.ncells * kern.Q * torch.empty((), dtype=kern.dtype).element_size()
+    checkpoint = resolve_checkpoint(state_bytes, nsteps, checkpoint, device=kern.device)
     seg = segment_length(nsteps, checkpoint)
     names, rho_in, rho_out = None, None, None
     if faces is not None:

@@ -62,3 +62,4 @@ def test_node_round_trip_continues_identically(tmp_path):
     assert torch.equal(a["f"].cpu(), b["f"].cpu())
     with pytest.raises(ValueError, match="node mismatch"):
         ck.load_state(out, {"other": node})
+

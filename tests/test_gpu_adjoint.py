@@ -94,7 +94,7 @@ def outlet_uz_aos(f_aos, Fz, mask):
     return torch.where(mask[:, :, -1] != 0, torch.zeros_like(uz), uz).mean()
 
 
-@pytest.mark.parametrize("checkpoint", ["sqrt", "all", 4])
+@pytest.mark.parametrize("checkpoint", ["sqrt", "all", 4, "auto"])
 def test_rollout_gradients_match_torch_autograd(checkpoint):
     from maddening_amd.adjoint import rollout
     from maddening_amd.lbm import LBMKernel
