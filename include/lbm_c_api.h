@@ -118,10 +118,11 @@ typedef struct lbm_desc {
 /* Two steps per launch (no counterpart in the reference: a launch schedule
  * of the same update). lbm_step runs pairs of its fused pull steps as one
  * launch that keeps the intermediate state in LDS, where the lattice allows
- * it: D3Q19, LBM_BB_NONE, LBM_F32 or LBM_F32_DEV, uniform force and tau, no
- * pressure faces, dims[2] <= 341 and 19 * cells < 2^31. By default only
- * where it measured faster (dims[2] == 256 and enough workgroups to fill
- * the GPU); PAIR_FORCE uses it wherever it applies,
+ * it: LBM_BB_NONE, LBM_F32 or LBM_F32_DEV, uniform force and tau, no
+ * pressure faces, Q * cells < 2^31, and for D3Q19 dims[2] <= 341. By
+ * default only where it measured faster (D3Q19: dims[2] == 256 and enough
+ * workgroups to fill the GPU; D2Q9: at least 2048^2 cells); PAIR_FORCE uses
+ * it wherever it applies,
  * PAIR_OFF never. Results are bit-identical either way. */
 #define LBM_FLAG_PAIR_OFF 16
 #define LBM_FLAG_PAIR_FORCE 32

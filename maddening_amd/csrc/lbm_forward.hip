@@ -547,11 +547,57 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   return true;
 }
 
-template <class T>
-int launch_pair(hipStream_t s, const PairArgs& pa) {
-  hipLaunchKernelGGL((k_pair<T>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s, pa);
-  return check_launch("k_pair launch");
+// D2Q9 plan: periodic, 32-bit storage, uniform force and tau, no pressure
+// faces, 9 planes addressable with 32-bit offsets; chunks of 32 rows. Without
+// LBM_FLAG_PAIR_FORCE only from 2048^2 cells on, where it measured faster
+// (f32dev, per two steps against two k_cell launches: 8192^2 1006 against
+// 1716 us, 4096^2 339 against 416, 2048^2 104 against 140, 2048 x 8192 350
+// against 419; 1024^2 and 512^2 lost at every chunk length).
+inline bool pair9_plan(const lbm_desc* d, const Geom& g, bool fields, Pair9Args* pa) {
+  if (d->flags & LBM_FLAG_PAIR_OFF) return false;
+  const bool force = (d->flags & LBM_FLAG_PAIR_FORCE) != 0;
+  if (d->q != 9 || d->bb_mode != LBM_BB_NONE || fields) return false;
+  if (d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
+  if (d->inlet_face > 0 || d->outlet_face > 0) return false;
+  if (9LL * g.plane >= (1LL << 31)) return false;
+  if (!force && g.plane < 2048LL * 2048) return false;
+  const int lb = 32;
+  const long long ctiles = (g.nc + 959) / 960;
+  const long long blocks = ctiles * ((g.nb + lb - 1) / lb);
+  if (blocks >= (1LL << 31)) return false;
+  std::memset(pa, 0, sizeof(*pa));
+  pa->nb = g.nb;
+  pa->nc = g.nc;
+  pa->pitch = (int)g.plane;
+  pa->ctiles = (int)ctiles;
+  pa->lb = lb;
+  pa->nblocks = (int)blocks;
+  pa->inv_tau = (float)(1.0 / d->tau);
+  pa->Fb = (float)d->force[0];
+  pa->Fc = (float)d->force[1];
+  return true;
 }
+
+// The two-step launch of each lattice: its argument block, plan and kernel.
+template <class L> struct PairOf;
+template <> struct PairOf<Lattice<19>> {
+  using Args = PairArgs;
+  static bool plan(const lbm_desc* d, const Geom& g, bool f, Args* a) { return pair_plan(d, g, f, a); }
+  template <class T>
+  static int launch(hipStream_t s, const Args& pa) {
+    hipLaunchKernelGGL((k_pair<T>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s, pa);
+    return check_launch("k_pair launch");
+  }
+};
+template <> struct PairOf<Lattice<9>> {
+  using Args = Pair9Args;
+  static bool plan(const lbm_desc* d, const Geom& g, bool f, Args* a) { return pair9_plan(d, g, f, a); }
+  template <class T>
+  static int launch(hipStream_t s, const Args& pa) {
+    hipLaunchKernelGGL((k_pair9<T>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s, pa);
+    return check_launch("k_pair9 launch");
+  }
+};
 
 // Launches of a pull sequence of nsteps steps: collide, npair two-step
 // launches, nsingle fused launches, stream + moments.
@@ -574,7 +620,8 @@ inline PullSchedule pull_schedule(int nsteps, bool pair) {
 template <class L, class T, int BB>
 int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
               const void* f_in, void* f_out, void* work, int nsteps,
-              float* rho, float* u, float* p, const PairArgs* pair = nullptr) {
+              float* rho, float* u, float* p,
+              const typename PairOf<L>::Args* pair = nullptr) {
   using C = typename Storage<T>::C;
   int rc;
   const PullSchedule sc = pull_schedule(nsteps, pair != nullptr);
@@ -585,13 +632,13 @@ int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
   k.src = f_in;
   k.dst = buf[(sc.launches - 1 - j++) & 1];
   if ((rc = launch_cell<L, T, BB, false, true, true>(s, g, k))) return rc;
-  if constexpr (L::Q == 19 && BB == BB_NONE && pair_storage<T>()) {
+  if constexpr (BB == BB_NONE && pair_storage<T>()) {
     if (sc.npair) {
-      PairArgs pa = *pair;
+      typename PairOf<L>::Args pa = *pair;
       for (int i = 0; i < sc.npair; ++i) {
         pa.src = k.dst;
         pa.dst = buf[(sc.launches - 1 - j++) & 1];
-        if ((rc = launch_pair<T>(s, pa))) return rc;
+        if ((rc = PairOf<L>::template launch<T>(s, pa))) return rc;
         k.dst = pa.dst;
       }
     }
@@ -664,8 +711,8 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
   if (scheme == LBM_SCHEME_PULL) {
     switch (d->bb_mode) {
       case LBM_BB_NONE: {
-        PairArgs pa;
-        const bool pair = pair_storage<T>() && pair_plan(d, g, ff || tf, &pa);
+        typename PairOf<L>::Args pa;
+        const bool pair = pair_storage<T>() && PairOf<L>::plan(d, g, ff || tf, &pa);
         return step_pull<L, T, BB_NONE>(s, gr, a, f_in, f_out, work, nsteps, rho, u, p,
                                         pair ? &pa : nullptr);
       }
@@ -741,14 +788,14 @@ struct PullPhaseOp {
       a.rho = rho; a.u = u; a.p = p;
     }
     if (phase == 3) {
-      PairArgs pa;
-      if (!pair_storage<T>() || !pair_plan(d, g, ff || tf, &pa))
+      typename PairOf<L>::Args pa;
+      if (!pair_storage<T>() || !PairOf<L>::plan(d, g, ff || tf, &pa))
         return fail(LBM_ERR_UNSUPPORTED,
                     "phase 3 (two steps per launch) does not apply to this lattice");
       if constexpr (pair_storage<T>()) {
         pa.src = src;
         pa.dst = dst;
-        return launch_pair<T>(s, pa);
+        return PairOf<L>::template launch<T>(s, pa);
       }
     }
     Geom gr = g;
@@ -893,7 +940,10 @@ int lbm_step_schedule(const lbm_desc* d, int nsteps, int with_fields, int* launc
     nl = nsteps + 1;  // one launch per step, then the moments pass
   } else {
     PairArgs pa;
-    const PullSchedule sc = pull_schedule(nsteps, pair_plan(d, g, with_fields != 0, &pa));
+    Pair9Args pa9;
+    const bool pair = d->q == 19 ? pair_plan(d, g, with_fields != 0, &pa)
+                                 : pair9_plan(d, g, with_fields != 0, &pa9);
+    const PullSchedule sc = pull_schedule(nsteps, pair);
     nl = sc.launches;
     np = sc.npair;
   }

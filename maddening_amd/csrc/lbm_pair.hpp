@@ -1,5 +1,5 @@
-// Two pull steps per launch for periodic D3Q19 lattices (temporal blocking
-// through LDS). Included by lbm_forward.hip.
+// Two pull steps per launch for periodic D3Q19 and D2Q9 lattices (temporal
+// blocking through LDS). Included by lbm_forward.hip and lbm_batch.hip.
 //
 // One fused pull launch reads the post-collision state g_n at x - e_q and
 // writes g_{n+1}(x) = C(f_{n+1}(x)): 152 B of HBM per cell update in float32.
@@ -239,6 +239,106 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   __shared__ float x[16][1024];
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   pair_march<T>(a, lb % a.btiles, lb / a.btiles, x);
+}
+
+// ---------------------------------------------------------------------------
+// D2Q9: two pull steps per launch. Rows along c are cut into tiles of 960
+// output cells (15 waves, so stores stay 64-cell aligned) with 32-cell halos
+// on either side; a 1024-thread workgroup marches along b over a chunk of
+// rows. Step 1 runs on all 1024 columns (the halos wrap periodically), step
+// 2 on the 960 inner ones, reading its left and right neighbours' step-1
+// values from LDS (24 KB: the six directions with e_c != 0). Same structure
+// and same bits as the D3Q19 march above, with rows in place of planes. At
+// 68 VGPRs and 24 KB it is light enough that the 16-wave workgroups of
+// neighbouring tiles share a CU's waves, which the D3Q19 form cannot.
+// ---------------------------------------------------------------------------
+struct Pair9Args {
+  const void* src;
+  void* dst;
+  int nb, nc;
+  int pitch;    // cells from one population plane to the next; 9 * pitch < 2^31
+  int ctiles;   // ceil(nc / 960)
+  int lb;       // rows per chunk
+  int nblocks;  // ctiles * ceil(nb / lb)
+  float inv_tau, Fb, Fc;
+};
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_pair9(Pair9Args a) {
+  using L = Lattice<9>;
+  using St = Storage<T>;
+  using C = float;
+  static_assert(sizeof(typename Storage<T>::C) == 4, "float compute only");
+  __shared__ float x[6][1024];  // q = 3 .. 8
+  const int ct = (int)blockIdx.x % a.ctiles;
+  const int b0 = ((int)blockIdx.x / a.ctiles) * a.lb;
+  const int b1 = b0 + a.lb < a.nb ? b0 + a.lb : a.nb;  // output rows [b0, b1)
+  const int nb = a.nb, nc = a.nc, P = a.pitch;
+  const int tid = (int)threadIdx.x;
+  const int cl = ct * 960 - 32 + tid;   // column before the periodic wrap
+  const int c = ((cl % nc) + nc) % nc;
+  const bool inner = tid >= 32 && tid < 992 && cl < nc;
+  const T* src = static_cast<const T*>(a.src);
+  T* dst = static_cast<T*>(a.dst);
+  auto fetch = [&](int p, C (&v)[9]) {
+    Nbr n;
+    n.init(0, (p + nb) % nb, c, 1, nb, nc);
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+      v[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
+  };
+  // own column's step-1 values: Z of row p - 1 (e_b = 0), P1 / P2 of rows
+  // p - 1 / p - 2 (e_b = +1); only the entries of those classes are used
+  C Z[9], P1[9], P2[9], nx[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) Z[q] = P1[q] = P2[q] = C(0);
+  fetch(b0 - 1, nx);
+  for (int p = b0 - 1; p <= b1; ++p) {
+    C d[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) d[q] = nx[q];
+    {
+      C drho, rho, ua, ub, uc;
+      moments<L, C>(d, C(0), a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C>(d, a.inv_tau, C(0), a.Fb, a.Fc, drho, rho, ua, ub, uc);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {  // the value a store and reload would give
+        const T s = St::template enc<L>(q, d[q]);
+        d[q] = St::template load<L>(&s, q);
+      }
+    }
+    const bool out = p > b0;
+    if (out) {
+#pragma unroll
+      for (int q = 3; q < 9; ++q)
+        x[q - 3][tid] = L::eb[q] < 0 ? d[q] : (L::eb[q] == 0 ? Z[q] : P2[q]);
+    }
+    __syncthreads();
+    if (p < b1) fetch(p + 1, nx);
+    if (out && inner) {
+      C e[9];
+      e[0] = Z[0];   // (p - 1, c)
+      e[1] = P2[1];  // e_b = +1: (p - 2, c)
+      e[2] = d[2];   // e_b = -1: (p, c)
+#pragma unroll
+      for (int q = 3; q < 9; ++q) e[q] = x[q - 3][tid - L::ec[q]];
+      C drho, rho, ua, ub, uc;
+      moments<L, C>(e, C(0), a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C>(e, a.inv_tau, C(0), a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      const int cell = (p - 1) * nc + c;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) St::template store<L>(dst + q * P + cell, q, e[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      if (L::eb[q] > 0) {
+        P2[q] = P1[q];
+        P1[q] = d[q];
+      }
+      if (L::eb[q] == 0) Z[q] = d[q];
+    }
+  }
 }
 
 }  // namespace mlbm

@@ -27,7 +27,7 @@ SHAPES = [(64, 80, 128), (33, 50, 64), (3, 2, 64), (1, 1, 64), (2, 1, 128), (130
 def _state(k, shape, seed):
     rng = np.random.default_rng(seed)
     rho = torch.from_numpy(1 + 0.01 * rng.standard_normal(shape)).float()
-    u = torch.from_numpy(0.03 * rng.standard_normal(shape + (3,))).float()
+    u = torch.from_numpy(0.03 * rng.standard_normal(shape + (len(shape),))).float()
     return k.equilibrium(rho, u)
 
 
@@ -151,3 +151,58 @@ def test_pair_launches_random_shapes_bitwise():
         torch.cuda.synchronize()
         for a, b in zip(*outs):
             assert torch.equal(a, b), (case, shape, storage, nsteps)
+
+
+# D2Q9: rows of 1 to 1920 cells (one or more 960-cell tiles, the last one
+# partial), row counts below and above a 32-row chunk
+SHAPES_2D = [(64, 1000), (7, 64), (3, 5), (1, 1), (33, 961), (100, 1920), (40, 2000)]
+
+
+@pytest.mark.parametrize("shape", SHAPES_2D)
+@pytest.mark.parametrize("storage", ["f32", "f32dev"])
+@pytest.mark.parametrize("nsteps", [2, 3, 6])
+def test_d2q9_pair_launches_equal_single_launches_bitwise(shape, storage, nsteps):
+    from maddening_amd.lbm import LBMKernel
+    print(f"D2Q9 pair vs single: {shape} {storage} nsteps={nsteps}", flush=True)
+    outs = []
+    for pair in ("force", "off"):
+        k = LBMKernel(shape, lattice="D2Q9", tau=0.7, storage=storage, scheme="pull", pair=pair)
+        assert k.schedule(nsteps)[1] == ((nsteps - 1) // 2 if pair == "force" else 0)
+        f = _state(k, shape, 4)
+        outs.append([t.cpu() for t in k.step(f, nsteps, force=[2e-5, -1e-5])])
+    torch.cuda.synchronize()
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
+def test_d2q9_pair_launches_match_oracle_f32():
+    from maddening_amd.lbm import LBMKernel
+    shape = (9, 70)
+    rng = np.random.default_rng(8)
+    rho0 = 1 + 0.01 * rng.standard_normal(shape)
+    u0 = 0.03 * rng.standard_normal(shape + (2,))
+    e, w, _ = orc.lattice(9)
+    f0 = orc.equilibrium(rho0, u0, e, w)
+    F = (1e-5, -2e-5)
+    k = LBMKernel(shape, lattice="D2Q9", tau=0.8, storage="f32", scheme="pull", pair="force")
+    f = k.from_aos(torch.from_numpy(np.ascontiguousarray(f0)).to("cuda", torch.float64))
+    f, rho, u, p = k.step(f, 7, force=list(F))
+    torch.cuda.synchronize()
+    ref = orc.run(f0, 7, tau=0.8, force=F)
+    g = k.to_aos(f, torch.float64).cpu().numpy()
+    assert np.max(np.abs(g - ref["f"])) / np.max(np.abs(ref["f"])) < 1e-5
+    assert np.max(np.abs(u.cpu().numpy() - ref["velocity"])) / np.max(np.abs(ref["velocity"])) < 1e-5
+
+
+def test_d2q9_auto_pairs_at_2048_squared_equal_single_launches():
+    from maddening_amd.lbm import LBMKernel
+    shape = (2048, 2048)
+    outs = []
+    for pair in ("auto", "off"):
+        k = LBMKernel(shape, lattice="D2Q9", tau=0.9, storage="f32dev", pair=pair)
+        assert k.schedule(9)[1] == (4 if pair == "auto" else 0)
+        f = _state(k, shape, 6)
+        outs.append([t.cpu() for t in k.step(f, 9, force=[1e-6, 0.0])])
+    torch.cuda.synchronize()
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)

@@ -5,10 +5,10 @@ import ctypes
 import pytest
 
 
-def _desc(dims, storage=4, bb=0, flags=0, faces=False):
+def _desc(dims, storage=4, bb=0, flags=0, faces=False, q=19):
     from maddening_amd import _lib
     d = _lib.LbmDesc()
-    d.q, d.storage, d.bb_mode, d.tau, d.flags = 19, storage, bb, 0.8, flags
+    d.q, d.storage, d.bb_mode, d.tau, d.flags = q, storage, bb, 0.8, flags
     for i, n in enumerate(dims):
         d.dims[i] = n
     if faces:
@@ -72,3 +72,16 @@ def test_phase_3_rejected_where_pairs_do_not_apply():
     assert rc == -2 and b"two steps per launch" in L.lbm_last_error()
     assert L.lbm_pull_phase(None, ctypes.byref(d), 4, ctypes.c_void_p(4096),
                             ctypes.c_void_p(8192), None, None, None, None, None, None, None) == -1
+
+
+def test_d2q9_pairs_from_2048_squared():
+    from maddening_amd import _lib
+    for dims in [(2048, 2048), (8192, 8192), (2048, 8192)]:
+        assert _schedule(_desc(dims, q=9), 200) == (2 + 99 + 1, 99)
+    for d in [_desc((1024, 1024), q=9), _desc((8192, 8192), q=9, storage=2),
+              _desc((8192, 8192), q=9, bb=2), _desc((8192, 8192), q=9, flags=_lib.LBM_FLAG_PAIR_OFF)]:
+        assert _schedule(d, 200) == (201, 0)
+    assert _schedule(_desc((2048, 2048), q=9), 200, with_fields=1) == (201, 0)
+    F = _lib.LBM_FLAG_PAIR_FORCE
+    for dims in [(1, 1), (7, 64), (33, 961)]:
+        assert _schedule(_desc(dims, q=9, flags=F), 9) == (6, 4)
