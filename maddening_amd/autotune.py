@@ -72,7 +72,7 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
     """Pick the launch shape for ``kern``'s lattice: the library's own choice,
     row blocks of each width (pull or push), flattened 256-cell tiles, the
     row-per-wave kernels at each legal vector width, two steps per launch
-    where the lattice allows it (D3Q19, periodic, 32-bit storage, rows of at
+    where the lattice allows it (periodic, 32-bit storage; D3Q19 rows of at
     most 341 cells), then XCD run lengths for the best of those. Applies and
     returns the winner with every candidate's ms per step."""
     dev = torch.cuda.get_device_name(kern.device)
@@ -93,7 +93,7 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
     if kern.bb == "none":
         shapes += [cand(0, v, "pull") for v in (1, 2, 4)
                    if kern.grid_shape[-1] % (64 * v) == 0 and not (v == 4 and kern.storage == "f64")]
-        if kern.Q == 19 and kern.storage in ("f32", "f32dev") and kern.grid_shape[-1] <= 341:
+        if kern.storage in ("f32", "f32dev") and (kern.Q == 9 or kern.grid_shape[-1] <= 341):
             shapes.append(cand(0, -1, "pull", pair="force"))
     f = kern.equilibrium()
     out = kern.empty()

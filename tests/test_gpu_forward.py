@@ -499,17 +499,17 @@ def test_tune_kernel_picks_a_candidate_and_keeps_results(tmp_path, monkeypatch):
     assert torch.equal(got, ref)
 
 
-def test_tune_kernel_times_two_step_launches(tmp_path, monkeypatch):
-    """For a periodic D3Q19 float32 lattice the tuner also times two steps per
+@pytest.mark.parametrize("lattice,shape", [("D3Q19", (24, 30, 100)), ("D2Q9", (300, 1000))])
+def test_tune_kernel_times_two_step_launches(tmp_path, monkeypatch, lattice, shape):
+    """For a periodic float32 lattice the tuner also times two steps per
     launch; whichever wins, the tuned kernel gives the default kernel's bits."""
     from maddening_amd import autotune
     monkeypatch.setattr(autotune, "_CACHE", str(tmp_path / "tune.json"))
-    shape = (24, 30, 100)
-    k = kernel(shape, "D3Q19", 0.8, storage="f32dev")
+    k = kernel(shape, lattice, 0.8, storage="f32dev")
     rec = autotune.tune_kernel(k, nsteps=4, reps=1)
     assert any(c["pair"] == "force" for c in rec["candidates"])
     assert k.pair == rec["pair"]
-    f0 = kernel(shape, "D3Q19", 0.8, storage="f32dev").equilibrium()
-    ref = kernel(shape, "D3Q19", 0.8, storage="f32dev", scheme="pull", pair="off").step(f0, 7)[0]
+    f0 = kernel(shape, lattice, 0.8, storage="f32dev").equilibrium()
+    ref = kernel(shape, lattice, 0.8, storage="f32dev", scheme="pull", pair="off").step(f0, 7)[0]
     k.scheme = "pull"
     assert torch.equal(k.step(f0, 7)[0], ref)
