@@ -91,8 +91,10 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
     shapes += [cand(bx, -1, sc) for bx in (64, 128, 256) for sc in ("pull", "push")]
     shapes.append(cand(-1, -1, "pull"))
     if kern.bb == "none":
+        # row-per-wave kernels exist for f32, f64 and 16-bit storage, not f32dev
         shapes += [cand(0, v, "pull") for v in (1, 2, 4)
-                   if kern.grid_shape[-1] % (64 * v) == 0 and not (v == 4 and kern.storage == "f64")]
+                   if kern.grid_shape[-1] % (64 * v) == 0 and kern.storage != "f32dev"
+                   and not (v == 4 and kern.storage == "f64")]
         if kern.storage in ("f32", "f32dev") and (kern.Q == 9 or kern.grid_shape[-1] <= 341):
             shapes.append(cand(0, -1, "pull", pair="force"))
     f = kern.equilibrium()

@@ -511,6 +511,8 @@ constexpr bool pair_storage() {
 // (f32, per two steps: 256x256x160 410 against 583 us, 128^3 108 against
 // 147, 160^3 209 against 240, 200x200x100 221 against 224, 320x320x192 1041
 // against 1110; rows of 128 with 22-43 workgroups 2-4x slower).
+constexpr long long kPairCUs = 256;  // compute units of one MI355X
+
 inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
   if (d->flags & LBM_FLAG_PAIR_OFF) return false;
   const bool force = (d->flags & LBM_FLAG_PAIR_FORCE) != 0;
@@ -524,8 +526,22 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   const long long btiles = (g.nb + tb - 1) / tb;
   int la = 128;
   auto blocks = [&](int l) { return btiles * ((g.na + l - 1) / l); };
-  while (!force && blocks(la) < 256 && la > 64) la /= 2;
-  if (!force && blocks(la) < 256) return false;
+  if (force) {
+    // one workgroup per CU: a launch takes ceil(blocks / CUs) rounds of
+    // (chunk + 2) planes each; take the chunk that minimises that (the
+    // measured best at 256^3: 128, 256x256x160: 64, 160^3: 32, 128^3: 16)
+    long long best = -1;
+    for (int l = 128; l >= 16; l /= 2) {
+      const long long cost = ((blocks(l) + kPairCUs - 1) / kPairCUs) * (l + 2);
+      if (best < 0 || cost < best) {
+        best = cost;
+        la = l;
+      }
+    }
+  } else {
+    while (blocks(la) < 256 && la > 64) la /= 2;
+    if (blocks(la) < 256) return false;
+  }
   if (blocks(la) >= (1LL << 31)) return false;
   std::memset(pa, 0, sizeof(*pa));
   pa->na = g.na;

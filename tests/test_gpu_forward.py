@@ -499,7 +499,8 @@ def test_tune_kernel_picks_a_candidate_and_keeps_results(tmp_path, monkeypatch):
     assert torch.equal(got, ref)
 
 
-@pytest.mark.parametrize("lattice,shape", [("D3Q19", (24, 30, 100)), ("D2Q9", (300, 1000))])
+@pytest.mark.parametrize("lattice,shape", [("D3Q19", (24, 30, 100)), ("D3Q19", (20, 24, 128)),
+                                           ("D2Q9", (300, 1000))])
 def test_tune_kernel_times_two_step_launches(tmp_path, monkeypatch, lattice, shape):
     """For a periodic float32 lattice the tuner also times two steps per
     launch; whichever wins, the tuned kernel gives the default kernel's bits."""
