@@ -564,7 +564,7 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
 }
 
 // D2Q9 plan: periodic, 32-bit storage, uniform force and tau, no pressure
-// faces, 9 planes addressable with 32-bit offsets; chunks of 32 rows. Without
+// faces, 9 planes addressable with 32-bit offsets. Without
 // LBM_FLAG_PAIR_FORCE only from 2048^2 cells on, where it measured faster
 // (f32dev, per two steps against two k_cell launches: 8192^2 1006 against
 // 1716 us, 4096^2 339 against 416, 2048^2 104 against 140, 2048 x 8192 350
@@ -577,9 +577,20 @@ inline bool pair9_plan(const lbm_desc* d, const Geom& g, bool fields, Pair9Args*
   if (d->inlet_face > 0 || d->outlet_face > 0) return false;
   if (9LL * g.plane >= (1LL << 31)) return false;
   if (!force && g.plane < 2048LL * 2048) return false;
-  const int lb = 32;
+  // chunk rows by the same rounds model as the D3Q19 plan (measured best:
+  // 8192^2 and 2048^2 32 rows, 4096^2 16 rows: 311 against 339 us)
   const long long ctiles = (g.nc + 959) / 960;
-  const long long blocks = ctiles * ((g.nb + lb - 1) / lb);
+  auto nblk = [&](int l) { return ctiles * ((g.nb + l - 1) / l); };
+  int lb = 32;
+  long long best = -1;
+  for (int l = 64; l >= 16; l /= 2) {
+    const long long cost = ((nblk(l) + kPairCUs - 1) / kPairCUs) * (l + 2);
+    if (best < 0 || cost < best) {
+      best = cost;
+      lb = l;
+    }
+  }
+  const long long blocks = nblk(lb);
   if (blocks >= (1LL << 31)) return false;
   std::memset(pa, 0, sizeof(*pa));
   pa->nb = g.nb;
