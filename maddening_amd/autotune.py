@@ -87,12 +87,14 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
         return {"block_x": block_x, "row_vec": row_vec, "scheme": scheme,
                 "xcd_remap": xcd_remap, "xcd_group": xcd_group, "pair": pair}
 
+    # the library's own choice, then one step per launch in each launch
+    # shape, then two steps per launch wherever the kernel covers the lattice
     shapes = [cand()]
-    shapes += [cand(bx, -1, sc) for bx in (64, 128, 256) for sc in ("pull", "push")]
-    shapes.append(cand(-1, -1, "pull"))
+    shapes += [cand(bx, -1, sc, pair="off") for bx in (64, 128, 256) for sc in ("pull", "push")]
+    shapes.append(cand(-1, -1, "pull", pair="off"))
     if kern.bb == "none":
         # row-per-wave kernels exist for f32, f64 and 16-bit storage, not f32dev
-        shapes += [cand(0, v, "pull") for v in (1, 2, 4)
+        shapes += [cand(0, v, "pull", pair="off") for v in (1, 2, 4)
                    if kern.grid_shape[-1] % (64 * v) == 0 and kern.storage != "f32dev"
                    and not (v == 4 and kern.storage == "f64")]
         if kern.storage in ("f32", "f32dev") and (kern.Q == 9 or kern.grid_shape[-1] <= 341):

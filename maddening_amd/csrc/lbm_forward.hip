@@ -498,21 +498,26 @@ constexpr bool pair_storage() {
   return std::is_same<T, float>::value || std::is_same<T, f32dev_t>::value;
 }
 
+constexpr long long kPairCUs = 256;  // compute units of one MI355X
+
 // Plan of the two-step launch for lattice g, or false where it does not
 // apply: D3Q19, periodic, 32-bit storage, uniform force and tau (`fields`:
 // the call passes a force or tau field), no pressure faces, rows of at most
 // 341 cells (three rows per 1024-thread workgroup), and 19 planes
-// addressable with 32-bit offsets.
-// Without LBM_FLAG_PAIR_FORCE it is used only where it was measured faster
-// than two fused launches: rows of 256 cells, chunks of at least 64 planes
-// and at least 256 workgroups, one per CU (256^3 f32dev: 626 us per launch
-// against 910 us for two k_cell launches, bit-identical results). Other
-// shapes won or lost depending on how many workgroups their chunks leave
-// (f32, per two steps: 256x256x160 410 against 583 us, 128^3 108 against
-// 147, 160^3 209 against 240, 200x200x100 221 against 224, 320x320x192 1041
-// against 1110; rows of 128 with 22-43 workgroups 2-4x slower).
-constexpr long long kPairCUs = 256;  // compute units of one MI355X
-
+// addressable with 32-bit offsets. The chunk length minimises rounds of
+// one-workgroup-per-CU launches x (chunk + 2) planes.
+// Without LBM_FLAG_PAIR_FORCE it is used only where that measured faster
+// than two fused launches: at least 2^21 cells, at least 64 cells along a
+// and b (thinner axes recompute their halo planes and rows several times
+// over), and rows of 64 to 256 cells in multiples of 32 (longer rows leave
+// a workgroup three rows, one of them its own output). Measured f32dev, ms
+// per step, two-step against the best one-step launch shape: 256^3 0.31
+// against 0.45, 512x256x128 0.338 against 0.473, 320x320x192 0.440 against
+// 0.574, 192^3 0.154 against 0.203, 256x256x160 0.210 against 0.302, 160^3
+// 0.100 against 0.125, 128^3 0.056 against 0.066, 64x256x128 0.056 against
+// 0.065, 256x128x64 0.057 against 0.0645; below that size or with other row
+// lengths it broke even or lost (200x200x100 0.117 against 0.115, 100^3
+// 0.050 against 0.031, 96^3 0.045 against 0.028).
 inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
   if (d->flags & LBM_FLAG_PAIR_OFF) return false;
   const bool force = (d->flags & LBM_FLAG_PAIR_FORCE) != 0;
@@ -521,12 +526,14 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   if (d->inlet_face > 0 || d->outlet_face > 0) return false;
   if (g.nc > 341) return false;
   if (19LL * g.plane >= (1LL << 31)) return false;
-  if (!force && g.nc != 256) return false;
+  if (!force && (g.plane < (1LL << 21) || g.na < 64 || g.nb < 64 || g.nc < 64 || g.nc > 256 ||
+                 g.nc % 32 != 0))
+    return false;
   const int tb = 1024 / g.nc - 2;
   const long long btiles = (g.nb + tb - 1) / tb;
   int la = 128;
   auto blocks = [&](int l) { return btiles * ((g.na + l - 1) / l); };
-  if (force) {
+  {
     // one workgroup per CU: a launch takes ceil(blocks / CUs) rounds of
     // (chunk + 2) planes each; take the chunk that minimises that (the
     // measured best at 256^3: 128, 256x256x160: 64, 160^3: 32, 128^3: 16)
@@ -538,9 +545,6 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
         la = l;
       }
     }
-  } else {
-    while (blocks(la) < 256 && la > 64) la /= 2;
-    if (blocks(la) < 256) return false;
   }
   if (blocks(la) >= (1LL << 31)) return false;
   std::memset(pa, 0, sizeof(*pa));

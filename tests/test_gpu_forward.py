@@ -509,6 +509,7 @@ def test_tune_kernel_times_two_step_launches(tmp_path, monkeypatch, lattice, sha
     k = kernel(shape, lattice, 0.8, storage="f32dev")
     rec = autotune.tune_kernel(k, nsteps=4, reps=1)
     assert any(c["pair"] == "force" for c in rec["candidates"])
+    assert any(c["pair"] == "off" for c in rec["candidates"])
     assert k.pair == rec["pair"]
     f0 = kernel(shape, lattice, 0.8, storage="f32dev").equilibrium()
     ref = kernel(shape, lattice, 0.8, storage="f32dev", scheme="pull", pair="off").step(f0, 7)[0]

@@ -123,6 +123,25 @@ def test_full_size_auto_pairs_equal_single_launches():
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("shape", [(128, 128, 128), (64, 256, 128), (256, 128, 64),
+                                   (80, 128, 224)])
+def test_auto_pairs_on_other_shapes_equal_single_launches(shape):
+    """Lattices the library runs two steps per launch by itself besides
+    256^3 (at least 2^21 cells, a and b of at least 64 cells, rows of 64-256
+    cells in multiples of 32): 9 steps equal 9 single launches bit for bit."""
+    from maddening_amd.lbm import LBMKernel
+    outs = []
+    for pair in ("auto", "off"):
+        k = LBMKernel(shape, lattice="D3Q19", tau=0.75, storage="f32dev", pair=pair)
+        assert k.schedule(9)[1] == (4 if pair == "auto" else 0)
+        f = _state(k, shape, 12)
+        outs.append([t.cpu() for t in k.step(f, 9, force=[1e-5, 0.0, -1e-5])])
+        del f
+    torch.cuda.synchronize()
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
 def test_pair_launches_random_shapes_bitwise():
     """Seeded random extents (rows of 1-341 cells, other axes 1-40), step
     counts 2-9, both 32-bit storages, random uniform force and tau: forced

@@ -33,13 +33,29 @@ def test_auto_pairs_at_256_cubed(storage, nsteps):
     assert nl == 2 + npair + (nsteps - 1) % 2
 
 
+# lattices where two-step launches measured faster than the best one-step
+# launch shape: at least 2^21 cells, a and b at least 64 cells, rows of 64
+# to 256 cells in multiples of 32
+@pytest.mark.parametrize("dims", [(128, 128, 128), (64, 256, 128), (256, 128, 64),
+                                  (160, 160, 160), (192, 192, 192), (256, 256, 160),
+                                  (320, 320, 192), (512, 256, 128), (64, 512, 64)])
+def test_auto_pairs_on_measured_winning_shapes(dims):
+    assert _schedule(_desc(dims), 9) == (6, 4)
+
+
 def test_auto_keeps_single_launches_where_pairs_measured_slower():
     from maddening_amd import _lib
-    # 16-bit and 64-bit storage, rows of 128 cells, too few workgroups,
-    # walls, pressure faces, force or tau fields, the push scheme, PAIR_OFF
+    # 16-bit and 64-bit storage; fewer than 2^21 cells (96^3 and 100^3
+    # measured slower with two-step launches); rows not a multiple of 32
+    # (200x200x100 broke even), shorter than 64 or longer than 256 cells;
+    # a or b thinner than 64 cells; walls, pressure faces, force or tau
+    # fields, the push scheme, PAIR_OFF
     cases = [_desc((256, 256, 256), storage=2), _desc((256, 256, 256), storage=3),
-             _desc((256, 256, 256), storage=1), _desc((128, 128, 128)),
-             _desc((16, 16, 256)), _desc((256, 256, 256), bb=1),
+             _desc((256, 256, 256), storage=1), _desc((96, 96, 96)), _desc((100, 100, 100)),
+             _desc((127, 128, 128)), _desc((16, 16, 256)), _desc((200, 200, 100)),
+             _desc((256, 256, 100)), _desc((1024, 64, 32)), _desc((256, 128, 288)),
+             _desc((256, 256, 320)), _desc((32768, 1, 64)), _desc((2, 4096, 256)),
+             _desc((63, 256, 256)), _desc((256, 63, 256)), _desc((256, 256, 256), bb=1),
              _desc((256, 256, 256), faces=True),
              _desc((256, 256, 256), flags=_lib.LBM_FLAG_PAIR_OFF)]
     for d in cases:
