@@ -490,12 +490,14 @@ int launch_push(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>:
 // Two steps per launch (k_pair, lbm_pair.hpp)
 // ---------------------------------------------------------------------------
 
-// Storage types k_pair is built for: 32-bit storage. With 16-bit storage it
-// measured slower than the row kernel (256^3: 569 us per two steps against
-// 2 x 215 us), because it is bound by load latency rather than by bytes.
-template <class T>
+// Storage types the two-step launches are built for: 32-bit storage, and
+// for D2Q9 16-bit deviation storage too. D3Q19 with 16-bit storage measured
+// slower than the row kernel (256^3: 569 us per two steps against 2 x 215
+// us), because k_pair is bound by load latency rather than by bytes.
+template <class L, class T>
 constexpr bool pair_storage() {
-  return std::is_same<T, float>::value || std::is_same<T, f32dev_t>::value;
+  return std::is_same<T, float>::value || std::is_same<T, f32dev_t>::value ||
+         (L::Q == 9 && (std::is_same<T, _Float16>::value || std::is_same<T, __bf16>::value));
 }
 
 constexpr long long kPairCUs = 256;  // compute units of one MI355X
@@ -567,9 +569,11 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   return true;
 }
 
-// D2Q9 plan: periodic, 32-bit storage, uniform force and tau, no pressure
-// faces, 9 planes addressable with 32-bit offsets. Without
-// LBM_FLAG_PAIR_FORCE only from 2048^2 cells on, where it measured faster
+// D2Q9 plan: periodic, 32-bit or 16-bit deviation storage (16-bit only
+// under LBM_FLAG_PAIR_FORCE: 8192^2 f16dev 71.6k MLUPS against 158.8k for
+// the row kernel), uniform force and tau, no pressure faces, 9 planes
+// addressable with 32-bit offsets. Without LBM_FLAG_PAIR_FORCE only 32-bit
+// storage and only from 2048^2 cells on, where it measured faster
 // (f32dev, per two steps against two k_cell launches: 8192^2 1006 against
 // 1716 us, 4096^2 339 against 416, 2048^2 104 against 140, 2048 x 8192 350
 // against 419; 1024^2 and 512^2 lost at every chunk length).
@@ -577,7 +581,8 @@ inline bool pair9_plan(const lbm_desc* d, const Geom& g, bool fields, Pair9Args*
   if (d->flags & LBM_FLAG_PAIR_OFF) return false;
   const bool force = (d->flags & LBM_FLAG_PAIR_FORCE) != 0;
   if (d->q != 9 || d->bb_mode != LBM_BB_NONE || fields) return false;
-  if (d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
+  if (d->storage == LBM_F64) return false;
+  if (!force && d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
   if (d->inlet_face > 0 || d->outlet_face > 0) return false;
   if (9LL * g.plane >= (1LL << 31)) return false;
   if (!force && g.plane < 2048LL * 2048) return false;
@@ -663,7 +668,7 @@ int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
   k.src = f_in;
   k.dst = buf[(sc.launches - 1 - j++) & 1];
   if ((rc = launch_cell<L, T, BB, false, true, true>(s, g, k))) return rc;
-  if constexpr (BB == BB_NONE && pair_storage<T>()) {
+  if constexpr (BB == BB_NONE && pair_storage<L, T>()) {
     if (sc.npair) {
       typename PairOf<L>::Args pa = *pair;
       for (int i = 0; i < sc.npair; ++i) {
@@ -743,7 +748,7 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
     switch (d->bb_mode) {
       case LBM_BB_NONE: {
         typename PairOf<L>::Args pa;
-        const bool pair = pair_storage<T>() && PairOf<L>::plan(d, g, ff || tf, &pa);
+        const bool pair = pair_storage<L, T>() && PairOf<L>::plan(d, g, ff || tf, &pa);
         return step_pull<L, T, BB_NONE>(s, gr, a, f_in, f_out, work, nsteps, rho, u, p,
                                         pair ? &pa : nullptr);
       }
@@ -820,10 +825,10 @@ struct PullPhaseOp {
     }
     if (phase == 3) {
       typename PairOf<L>::Args pa;
-      if (!pair_storage<T>() || !PairOf<L>::plan(d, g, ff || tf, &pa))
+      if (!pair_storage<L, T>() || !PairOf<L>::plan(d, g, ff || tf, &pa))
         return fail(LBM_ERR_UNSUPPORTED,
                     "phase 3 (two steps per launch) does not apply to this lattice");
-      if constexpr (pair_storage<T>()) {
+      if constexpr (pair_storage<L, T>()) {
         pa.src = src;
         pa.dst = dst;
         return PairOf<L>::template launch<T>(s, pa);

@@ -248,9 +248,13 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
 // rows. Step 1 runs on all 1024 columns (the halos wrap periodically), step
 // 2 on the 960 inner ones, reading its left and right neighbours' step-1
 // values from LDS (24 KB: the six directions with e_c != 0). Same structure
-// and same bits as the D3Q19 march above, with rows in place of planes. At
-// 68 VGPRs and 24 KB it is light enough that the 16-wave workgroups of
-// neighbouring tiles share a CU's waves, which the D3Q19 form cannot.
+// and same bits as the D3Q19 march above, with rows in place of planes. It
+// takes 68 VGPRs (7 waves per SIMD: one 16-wave workgroup per CU); capped at
+// 64 so that two fit, it spills 3 and is no faster (8192^2: 133.4k against
+// 133.5k MLUPS, 2048^2: 81.7k against 85.0k). With 16-bit storage each lane
+// moves 2-byte values, and it is 2.2x slower than the row kernels that move
+// 2-4 cells per lane (8192^2 f16dev: 71.6k against 158.8k MLUPS), so it runs
+// there only under LBM_FLAG_PAIR_FORCE.
 // ---------------------------------------------------------------------------
 struct Pair9Args {
   const void* src;

@@ -178,7 +178,7 @@ SHAPES_2D = [(64, 1000), (7, 64), (3, 5), (1, 1), (33, 961), (100, 1920), (40, 2
 
 
 @pytest.mark.parametrize("shape", SHAPES_2D)
-@pytest.mark.parametrize("storage", ["f32", "f32dev"])
+@pytest.mark.parametrize("storage", ["f32", "f32dev", "f16dev", "bf16dev"])
 @pytest.mark.parametrize("nsteps", [2, 3, 6])
 def test_d2q9_pair_launches_equal_single_launches_bitwise(shape, storage, nsteps):
     from maddening_amd.lbm import LBMKernel

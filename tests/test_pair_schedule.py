@@ -101,3 +101,13 @@ def test_d2q9_pairs_from_2048_squared():
     F = _lib.LBM_FLAG_PAIR_FORCE
     for dims in [(1, 1), (7, 64), (33, 961)]:
         assert _schedule(_desc(dims, q=9, flags=F), 9) == (6, 4)
+
+
+def test_d2q9_16_bit_pairs_only_when_forced():
+    from maddening_amd import _lib
+    F = _lib.LBM_FLAG_PAIR_FORCE
+    for storage in (2, 3):  # f16dev, bf16dev
+        assert _schedule(_desc((8192, 8192), storage=storage, q=9), 200) == (201, 0)
+        assert _schedule(_desc((64, 100), storage=storage, q=9, flags=F), 9) == (6, 4)
+    # never float64
+    assert _schedule(_desc((64, 100), storage=1, q=9, flags=F), 9) == (10, 0)
