@@ -70,13 +70,16 @@ def _apply(kern: LBMKernel, cand: dict) -> None:
 
 def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool = True) -> dict:
     """Pick the launch shape for ``kern``'s lattice: the library's own choice,
-    row blocks of each width (pull or push), flattened 256-cell tiles, the
-    row-per-wave kernels at each legal vector width, two steps per launch
-    where the lattice allows it (periodic, 32-bit storage; D3Q19 rows of at
-    most 341 cells), then XCD run lengths for the best of those. Applies and
-    returns the winner with every candidate's ms per step."""
+    then one step per launch in row blocks of each width (pull or push),
+    flattened 256-cell tiles and the row-per-wave kernels at each legal
+    vector width, two steps per launch where the lattice allows it
+    (periodic, 32-bit storage; D3Q19 rows of at most 341 cells), then XCD
+    run lengths for the best of those. Applies and returns the winner with
+    every candidate's ms per step. The one-step candidates carry
+    ``pair="off"``, so a cached one-step winner stays one step per launch
+    whatever the library's own choice is."""
     dev = torch.cuda.get_device_name(kern.device)
-    key = f"kernel3|{dev}|{kern.lat.name}|{kern.grid_shape}|{kern.storage}|{kern.bb}"
+    key = f"kernel4|{dev}|{kern.lat.name}|{kern.grid_shape}|{kern.storage}|{kern.bb}"
     if use_cache:
         hit = _load().get(key)
         if hit:
