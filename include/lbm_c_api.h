@@ -120,11 +120,11 @@ typedef struct lbm_desc {
  * launch that keeps the intermediate state in LDS, where the lattice allows
  * it: LBM_BB_NONE, LBM_F32 or LBM_F32_DEV (D2Q9 under PAIR_FORCE also
  * LBM_F16_DEV and LBM_BF16_DEV), uniform force and tau, no pressure faces,
- * Q * cells < 2^31, and for D3Q19 dims[2] <= 341. By
- * default only where it measured faster (D3Q19: at least 2^21 cells,
- * dims[0] and dims[1] at least 64, dims[2] one of 64, 96, ..., 256; D2Q9:
- * at least 2048^2 cells); PAIR_FORCE uses it wherever it applies, PAIR_OFF
- * never. Results are bit-identical either way. */
+ * Q * cells < 2^31. By default only where it measured faster (D3Q19: at
+ * least 2^21 cells, dims[0] and dims[1] at least 64, dims[2] one of 64, 96,
+ * ..., 256 or above 341; D2Q9: at least 2048^2 cells); PAIR_FORCE uses it
+ * wherever it applies, PAIR_OFF never. Results are bit-identical either
+ * way. */
 #define LBM_FLAG_PAIR_OFF 16
 #define LBM_FLAG_PAIR_FORCE 32
 

@@ -73,7 +73,7 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
     then one step per launch in row blocks of each width (pull or push),
     flattened 256-cell tiles and the row-per-wave kernels at each legal
     vector width, two steps per launch where the lattice allows it
-    (periodic, 32-bit storage; D3Q19 rows of at most 341 cells), then XCD
+    (periodic, 32-bit storage), then XCD
     run lengths for the best of those. Applies and returns the winner with
     every candidate's ms per step. The one-step candidates carry
     ``pair="off"``, so a cached one-step winner stays one step per launch
@@ -100,7 +100,7 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
         shapes += [cand(0, v, "pull", pair="off") for v in (1, 2, 4)
                    if kern.grid_shape[-1] % (64 * v) == 0 and kern.storage != "f32dev"
                    and not (v == 4 and kern.storage == "f64")]
-        if kern.storage in ("f32", "f32dev") and (kern.Q == 9 or kern.grid_shape[-1] <= 341):
+        if kern.storage in ("f32", "f32dev"):
             shapes.append(cand(0, -1, "pull", pair="force"))
     f = kern.equilibrium()
     out = kern.empty()

@@ -156,6 +156,7 @@ __global__ void __launch_bounds__(1024)
   a.dst = up(e.buf[parity ^ 1]);
   a.na = u32(e.na); a.nb = u32(e.nb); a.nc = u32(e.nc);
   a.pitch = u32((int)e.pitch);
+  a.w = a.nc; a.tc = a.nc; a.halo = 0; a.ctiles = 1;  // whole rows
   a.tb = u32(e.tb); a.btiles = u32(e.btiles); a.la = u32(e.la);
   a.nblocks = 0; a.remap = 0;
   a.inv_tau = uf(e.inv_tau);

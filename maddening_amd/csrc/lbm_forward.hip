@@ -501,18 +501,29 @@ constexpr bool pair_storage() {
 }
 
 constexpr long long kPairCUs = 256;  // compute units of one MI355X
+// Rows of up to this many cells are held whole by a workgroup (at least
+// three of them); longer rows are cut into tiles of kPairTile columns plus
+// one halo column each side.
+constexpr long long kPairWholeRows = 341;
+constexpr int kPairTile = 64;
 
 // Plan of the two-step launch for lattice g, or false where it does not
 // apply: D3Q19, periodic, 32-bit storage, uniform force and tau (`fields`:
-// the call passes a force or tau field), no pressure faces, rows of at most
-// 341 cells (three rows per 1024-thread workgroup), and 19 planes
-// addressable with 32-bit offsets. The chunk length minimises rounds of
-// one-workgroup-per-CU launches x (chunk + 2) planes.
+// the call passes a force or tau field), no pressure faces, and 19 planes
+// addressable with 32-bit offsets. Rows of at most 341 cells are held whole
+// (three or more per 1024-thread workgroup); longer rows are cut into
+// 64-column tiles. Tiling rows that fit whole measured slower (256^3 f32dev,
+// MLUPS: whole rows 54.5k, tiles of 32 / 64 / 128 columns 41.1k / 43.5k /
+// 39.5k). The chunk length minimises rounds of one-workgroup-per-CU
+// launches x (chunk + 2) planes.
 // Without LBM_FLAG_PAIR_FORCE it is used only where that measured faster
 // than two fused launches: at least 2^21 cells, at least 64 cells along a
 // and b (thinner axes recompute their halo planes and rows several times
-// over), and rows of 64 to 256 cells in multiples of 32 (longer rows leave
-// a workgroup three rows, one of them its own output). Measured f32dev, ms
+// over), and rows either of 64 to 256 cells in multiples of 32 (whole rows
+// of 288-341 cells leave a workgroup three rows, one of them its own
+// output) or longer than 341 cells (tiled: 128x128x512 39.6k against 36.5k
+// MLUPS, 256x256x512 39.2k against 34.9k, 64x64x2048 40.1k against 37.1k,
+// 96x96x1000 38.5k against 35.8k). Measured whole rows, f32dev, ms
 // per step, two-step against the best one-step launch shape: 256^3 0.31
 // against 0.45, 512x256x128 0.338 against 0.473, 320x320x192 0.440 against
 // 0.574, 192^3 0.154 against 0.203, 256x256x160 0.210 against 0.302, 160^3
@@ -526,15 +537,20 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   if (d->q != 19 || d->bb_mode != LBM_BB_NONE || fields) return false;
   if (d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
   if (d->inlet_face > 0 || d->outlet_face > 0) return false;
-  if (g.nc > 341) return false;
   if (19LL * g.plane >= (1LL << 31)) return false;
-  if (!force && (g.plane < (1LL << 21) || g.na < 64 || g.nb < 64 || g.nc < 64 || g.nc > 256 ||
-                 g.nc % 32 != 0))
+  if (!force && (g.plane < (1LL << 21) || g.na < 64 || g.nb < 64 || g.nc < 64 ||
+                 (g.nc <= kPairWholeRows && (g.nc > 256 || g.nc % 32 != 0))))
     return false;
-  const int tb = 1024 / g.nc - 2;
+  // whole rows up to kPairWholeRows cells, else c-tiles of kPairTile columns
+  const bool tiled = g.nc > kPairWholeRows && g.nc >= kPairTile;
+  const int w = tiled ? kPairTile + 2 : (int)g.nc;
+  if (w > 341) return false;
+  const int tc = tiled ? kPairTile : (int)g.nc;
+  const long long ctiles = (g.nc + tc - 1) / tc;
+  const int tb = 1024 / w - 2;
   const long long btiles = (g.nb + tb - 1) / tb;
   int la = 128;
-  auto blocks = [&](int l) { return btiles * ((g.na + l - 1) / l); };
+  auto blocks = [&](int l) { return ctiles * btiles * ((g.na + l - 1) / l); };
   {
     // one workgroup per CU: a launch takes ceil(blocks / CUs) rounds of
     // (chunk + 2) planes each; take the chunk that minimises that (the
@@ -554,6 +570,10 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   pa->nb = g.nb;
   pa->nc = g.nc;
   pa->pitch = (int)g.plane;
+  pa->w = w;
+  pa->tc = tc;
+  pa->halo = tiled ? 1 : 0;
+  pa->ctiles = (int)ctiles;
   pa->tb = tb;
   pa->btiles = (int)btiles;
   pa->la = la;

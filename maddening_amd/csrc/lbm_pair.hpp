@@ -7,7 +7,9 @@
 // per two updates. A 1024-thread workgroup owns R = 1024 / nc whole rows of
 // the contiguous axis c (nc <= 341, so R >= 3; threads past R * nc only join
 // the barriers), rows b0 - 1 .. b0 + R - 2 of axis b, and marches along axis
-// a over a chunk of planes:
+// a over a chunk of planes. Longer rows are cut into c-tiles of tc columns
+// with one halo column each side, which compute step 1 only; the rows of a
+// workgroup are then those tiles (w = tc + 2 cells, R = 1024 / w):
 //
 //   plane p:  step 1 at (p, row, c) for all R rows: gather g_n from HBM
 //             (the next plane's gather is issued before step 2, so its
@@ -22,8 +24,9 @@
 // that cross to another thread (all but q = 0, 1, 2, whose source is the
 // thread's own column) pass through 64 KB of LDS, written per plane:
 // e_a = -1 values of plane p, e_a = 0 values of plane p - 1 and e_a = +1
-// values of plane p - 2. Because a workgroup spans whole rows, the periodic
-// wrap along c stays inside it; the wrap along a and b is index arithmetic.
+// values of plane p - 2. With whole rows the periodic wrap along c stays
+// inside the workgroup; with c-tiles, and along a and b, it is index
+// arithmetic.
 //
 // Each update is the same inlined moments + collision as k_cell's, and step
 // 1's results pass through the storage type's encode/decode exactly as a
@@ -42,21 +45,29 @@ namespace mlbm {
 struct PairArgs {
   const void* src;
   void* dst;
-  int na, nb, nc;  // nc <= 341
+  int na, nb, nc;
   int pitch;       // cells from one population plane to the next (>= na * nb * nc);
                    // 19 * pitch < 2^31, so offsets fit 32 bits
-  int tb;          // output rows per workgroup = 1024 / nc - 2
+  int w;           // cells per workgroup row: nc (whole rows, nc <= 341) or a
+                   // c-tile of tc output columns with one halo column each side
+  int tc;          // output columns per row: nc, or the tile width (nc >= tc)
+  int halo;        // 0 for whole rows, 1 for c-tiles
+  int ctiles;      // 1, or ceil(nc / tc)
+  int tb;          // output rows per workgroup = 1024 / w - 2
   int btiles;      // ceil(nb / tb)
   int la;          // planes per chunk
-  int nblocks;     // btiles * ceil(na / la)
+  int nblocks;     // ctiles * btiles * ceil(na / la)
   int remap;       // xcd_remap run length (0: identity)
   float inv_tau, Fa, Fb, Fc;
 };
 
-// The march of one workgroup over b-tile bt and a-chunk `chunk` of lattice a;
-// x is the workgroup's 64 KB exchange buffer.
+// The march of one workgroup over c-tile ct, b-tile bt and a-chunk `chunk`
+// of lattice a; x is the workgroup's 64 KB exchange buffer. With whole rows
+// (w == nc) the periodic wrap along c stays inside the workgroup; with
+// c-tiles the halo columns compute step 1 only, and step 2 of the tc inner
+// columns reads them.
 template <class T>
-__device__ __forceinline__ void pair_march(const PairArgs& a, int bt, int chunk,
+__device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, int chunk,
                                            float (&x)[16][1024]) {
   using L = Lattice<19>;
   using St = Storage<T>;
@@ -65,13 +76,17 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int bt, int chunk,
   const int a0 = chunk * a.la;
   const int a1 = a0 + a.la < a.na ? a0 + a.la : a.na;  // output planes [a0, a1)
   const int b0 = bt * a.tb;
-  const int nc = a.nc, na = a.na, nb = a.nb, P = a.pitch;
+  const int nc = a.nc, na = a.na, nb = a.nb, P = a.pitch, w = a.w;
   const int tid = (int)threadIdx.x;
-  const int r = tid / nc;
-  const int c = tid - r * nc;
+  const int r = tid / w;
+  const int j = tid - r * w;
+  // column before the periodic wrap: -1 .. ctiles * tc, below 2 * nc as nc >= tc
+  const int cl = ct * a.tc - a.halo + j;
+  const int c = cl < 0 ? cl + nc : (cl >= nc ? cl - nc : cl);
   const bool active = r < a.tb + 2;
   const int b = (b0 - 1 + r + nb) % nb;
-  const bool inner = r >= 1 && r <= a.tb && b0 + r - 1 < nb;
+  const bool inner = r >= 1 && r <= a.tb && b0 + r - 1 < nb && j >= a.halo &&
+                     j < a.halo + a.tc && cl < nc;
   const T* src = static_cast<const T*>(a.src);
   T* dst = static_cast<T*>(a.dst);
 
@@ -118,9 +133,11 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int bt, int chunk,
       e[2] = d[2];   // (p, b, c)
 #pragma unroll
       for (int q = 3; q < 19; ++q) {
-        int cc = c - L::ec[q];  // periodic along c, inside the workgroup's rows
-        cc = cc < 0 ? cc + nc : (cc >= nc ? cc - nc : cc);
-        e[q] = x[q - 3][(r - L::eb[q]) * nc + cc];
+        // the row's neighbour column: periodic inside whole rows; an inner
+        // column of a c-tile (j in 1 .. tc) has both neighbours in its tile
+        int cc = j - L::ec[q];
+        cc = cc < 0 ? cc + w : (cc >= w ? cc - w : cc);
+        e[q] = x[q - 3][(r - L::eb[q]) * w + cc];
       }
       C drho, rho, ua, ub, uc;
       moments<L, C>(e, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
@@ -238,7 +255,8 @@ template <class T>
 __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   __shared__ float x[16][1024];
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
-  pair_march<T>(a, lb % a.btiles, lb / a.btiles, x);
+  const int rest = lb / a.ctiles;
+  pair_march<T>(a, lb - rest * a.ctiles, rest % a.btiles, rest / a.btiles, x);
 }
 
 // ---------------------------------------------------------------------------

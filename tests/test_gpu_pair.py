@@ -17,11 +17,13 @@ from oracle import lbm_numpy as orc
 
 pytestmark = pytest.mark.gpu
 
-# rows of 1 to 341 cells (1024 / nc rows per workgroup, at least three);
+# whole rows of 1 to 341 cells (1024 / nc rows per workgroup, at least
+# three) and longer rows cut into 64-column tiles (the last one partial);
 # partial b-tiles (nb not a multiple of the rows per workgroup - 2), partial
 # a-chunks, axes of 1-3 cells
 SHAPES = [(64, 80, 128), (33, 50, 64), (3, 2, 64), (1, 1, 64), (2, 1, 128), (130, 7, 256),
-          (5, 3, 256), (7, 9, 100), (4, 5, 341), (2, 2, 3), (1, 1, 1), (40, 37, 160)]
+          (5, 3, 256), (7, 9, 100), (4, 5, 341), (2, 2, 3), (1, 1, 1), (40, 37, 160),
+          (3, 4, 342), (5, 14, 512), (2, 1, 1000), (9, 27, 700), (1, 2, 1024)]
 
 
 def _state(k, shape, seed):
@@ -124,11 +126,12 @@ def test_full_size_auto_pairs_equal_single_launches():
 
 
 @pytest.mark.parametrize("shape", [(128, 128, 128), (64, 256, 128), (256, 128, 64),
-                                   (80, 128, 224)])
+                                   (80, 128, 224), (64, 64, 640), (64, 70, 1000)])
 def test_auto_pairs_on_other_shapes_equal_single_launches(shape):
     """Lattices the library runs two steps per launch by itself besides
-    256^3 (at least 2^21 cells, a and b of at least 64 cells, rows of 64-256
-    cells in multiples of 32): 9 steps equal 9 single launches bit for bit."""
+    256^3 (at least 2^21 cells, a and b of at least 64 cells, whole rows of
+    64-256 cells in multiples of 32 or rows of more than 341 cells in
+    64-column tiles): 9 steps equal 9 single launches bit for bit."""
     from maddening_amd.lbm import LBMKernel
     outs = []
     for pair in ("auto", "off"):
@@ -143,13 +146,14 @@ def test_auto_pairs_on_other_shapes_equal_single_launches(shape):
 
 
 def test_pair_launches_random_shapes_bitwise():
-    """Seeded random extents (rows of 1-341 cells, other axes 1-40), step
-    counts 2-9, both 32-bit storages, random uniform force and tau: forced
-    two-step launches equal one launch per step bit for bit."""
+    """Seeded random extents (rows of 1-1200 cells, whole or in 64-column
+    tiles; other axes 1-40), step counts 2-9, both 32-bit storages, random
+    uniform force and tau: forced two-step launches equal one launch per
+    step bit for bit."""
     from maddening_amd.lbm import LBMKernel
     rng = np.random.default_rng(2024)
-    for case in range(24):
-        nc = int(rng.integers(1, 342))
+    for case in range(32):
+        nc = int(rng.integers(1, 342)) if case < 24 else int(rng.integers(342, 1201))
         na = int(rng.integers(1, 41))
         nb = int(rng.integers(1, 41))
         while na * nb * nc > 200_000:

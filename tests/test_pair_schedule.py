@@ -34,11 +34,14 @@ def test_auto_pairs_at_256_cubed(storage, nsteps):
 
 
 # lattices where two-step launches measured faster than the best one-step
-# launch shape: at least 2^21 cells, a and b at least 64 cells, rows of 64
-# to 256 cells in multiples of 32
+# launch shape: at least 2^21 cells, a and b at least 64 cells, whole rows
+# of 64 to 256 cells in multiples of 32, or rows of more than 341 cells in
+# 64-column tiles
 @pytest.mark.parametrize("dims", [(128, 128, 128), (64, 256, 128), (256, 128, 64),
                                   (160, 160, 160), (192, 192, 192), (256, 256, 160),
-                                  (320, 320, 192), (512, 256, 128), (64, 512, 64)])
+                                  (320, 320, 192), (512, 256, 128), (64, 512, 64),
+                                  (128, 128, 512), (256, 256, 512), (64, 64, 2048),
+                                  (96, 96, 1000), (64, 96, 342)])
 def test_auto_pairs_on_measured_winning_shapes(dims):
     assert _schedule(_desc(dims), 9) == (6, 4)
 
@@ -55,6 +58,7 @@ def test_auto_keeps_single_launches_where_pairs_measured_slower():
              _desc((127, 128, 128)), _desc((16, 16, 256)), _desc((200, 200, 100)),
              _desc((256, 256, 100)), _desc((1024, 64, 32)), _desc((256, 128, 288)),
              _desc((256, 256, 320)), _desc((32768, 1, 64)), _desc((2, 4096, 256)),
+             _desc((64, 64, 500)), _desc((32, 128, 1024)), _desc((128, 16, 1024)),
              _desc((63, 256, 256)), _desc((256, 63, 256)), _desc((256, 256, 256), bb=1),
              _desc((256, 256, 256), faces=True),
              _desc((256, 256, 256), flags=_lib.LBM_FLAG_PAIR_OFF)]
@@ -66,16 +70,16 @@ def test_auto_keeps_single_launches_where_pairs_measured_slower():
     assert _schedule(d, 100) == (101, 0)
 
 
-def test_force_flag_reaches_rows_of_up_to_341_cells():
+def test_force_flag_reaches_every_row_length():
     from maddening_amd import _lib
     F = _lib.LBM_FLAG_PAIR_FORCE
+    # whole rows up to 341 cells, 64-column tiles beyond
     for dims in [(1, 1, 64), (33, 50, 64), (128, 128, 128), (3, 2, 256), (8, 8, 96),
-                 (2, 3, 341), (1, 1, 1)]:
+                 (2, 3, 341), (1, 1, 1), (8, 8, 342), (8, 8, 512), (2, 1, 2000)]:
         assert _schedule(_desc(dims, flags=F), 9) == (6, 4)
-    # not for rows of more than 341 cells (fewer than three per workgroup),
-    # nor 16-bit storage
-    for d in [_desc((8, 8, 342), flags=F), _desc((8, 8, 512), flags=F),
-              _desc((8, 8, 64), storage=2, flags=F)]:
+    # not 16-bit or 64-bit storage
+    for d in [_desc((8, 8, 64), storage=2, flags=F), _desc((8, 8, 512), storage=3, flags=F),
+              _desc((8, 8, 64), storage=1, flags=F)]:
         assert _schedule(d, 9) == (10, 0)
 
 
