@@ -503,7 +503,8 @@ constexpr bool pair_storage() {
 constexpr long long kPairCUs = 256;  // compute units of one MI355X
 // Rows of up to this many cells are held whole by a workgroup (at least
 // three of them); longer rows are cut into tiles of kPairTile columns plus
-// one halo column each side.
+// one halo column each side. 64 is the width that beat one step per launch
+// on every long-row lattice measured (32, 62, 126 and 128 each lost on one).
 constexpr long long kPairWholeRows = 341;
 constexpr int kPairTile = 64;
 
