@@ -507,6 +507,50 @@ def d2q9_leg(dev, n=8192, nsteps=200, reps=3):
     return out
 
 
+def long_rows_leg(dev, shape=(128, 128, 1024), nsteps=200, reps=3):
+    """D3Q19 periodic Taylor-Green with rows longer than one workgroup holds
+    whole (1024 cells), f32dev: the library's own choice (two steps per
+    launch in 64-column tiles) against one step per launch. Same bits either
+    way; 152 B per cell update."""
+    import numpy as np
+    import torch
+    from maddening_amd.lbm import LBMKernel
+    # Taylor-Green vortex with one period along each axis (k = 2 pi / n per
+    # axis), unit density
+    X, Y, Z = np.meshgrid(*(2 * np.pi * np.arange(n) / n for n in shape), indexing="ij")
+    u = np.stack([0.05 * np.sin(X) * np.cos(Y) * np.cos(Z),
+                  -0.05 * np.cos(X) * np.sin(Y) * np.cos(Z), np.zeros_like(X)], axis=-1)
+    rho = torch.ones(shape, dtype=torch.float32, device=dev)
+    u = torch.from_numpy(u).float().to(dev)
+    del X, Y, Z
+    cells = shape[0] * shape[1] * shape[2]
+    out = {"grid": list(shape), "steps": nsteps}
+    for pair in ("auto", "off"):
+        k = LBMKernel(shape, lattice="D3Q19", tau=0.8, storage="f32dev", pair=pair, device=dev)
+        f0 = k.equilibrium(rho, u)
+        work = k.workspace(nsteps)
+        res = k.empty()
+        k.step(f0, 5, out=res, work=work, moments=False)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            k.step(f0, nsteps, out=res, work=work, moments=False)
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1) / 1e3)
+        t = statistics.median(times)
+        out["MLUPS" if pair == "auto" else "single_pass_MLUPS"] = round(cells * nsteps / t / 1e6, 1)
+        if pair == "auto":
+            out["two_step_launches"] = k.schedule(nsteps)[1]
+            out["algorithmic_GBs"] = round(152 * cells * nsteps / t / 1e9, 1)
+        del k, f0, work, res
+        torch.cuda.empty_cache()
+    out["speedup"] = round(out["MLUPS"] / out["single_pass_MLUPS"], 3)
+    return out
+
+
 def pipe_leg(dev, shape=(256, 128, 128), nsteps=200, reps=3):
     """LBMPipeNode (lbm_pipe.py:751-879) on the GPU: single phase with the
     D3Q7 tracer, and Shan-Chen multiphase, float32, propeller + gravity.
@@ -574,11 +618,12 @@ def main():
                          "the default: same bytes as f32, 10x smaller rounding) or f32")
     ap.add_argument("--variants", default="f32,f16dev,bf16dev",
                     help="extra storage types to time (comma list, '' for none)")
-    ap.add_argument("--legs", default="precision,adjoint,batch,pipe,d2q9",
+    ap.add_argument("--legs", default="precision,adjoint,batch,pipe,d2q9,long_rows",
                     help="extra measurements: precision (storage types vs float64 at "
                          "100/1000/5000 steps), adjoint (forward+adjoint ms/step, gradient "
                          "vs central differences), batch (64 uneven lattices), pipe "
-                         "(LBMPipeNode single phase and multiphase), d2q9 (8192^2 D2Q9)")
+                         "(LBMPipeNode single phase and multiphase), d2q9 (8192^2 D2Q9), "
+                         "long_rows (D3Q19 128x128x1024, rows in 64-column tiles)")
     ap.add_argument("--pair", default="auto", choices=["auto", "off", "force"],
                     help="two pull steps per launch (LBM_FLAG_PAIR_*)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -719,6 +764,8 @@ def main():
         result["pipe"] = pipe_leg(dev)
     if "d2q9" in legs:
         result["d2q9"] = d2q9_leg(dev)
+    if "long_rows" in legs:
+        result["long_rows"] = long_rows_leg(dev)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(result), flush=True)
