@@ -78,10 +78,10 @@ def test_pair_launches_match_oracle_f32(shape):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f32dev"])
-def test_pull_phase_3_equals_two_phase_1(storage):
+@pytest.mark.parametrize("shape", [(9, 20, 128), (3, 5, 700)])  # whole rows, 64-column tiles
+def test_pull_phase_3_equals_two_phase_1(storage, shape):
     from maddening_amd import _lib
     from maddening_amd.lbm import LBMKernel, _stream
-    shape = (9, 20, 128)
     k = LBMKernel(shape, lattice="D3Q19", tau=0.9, storage=storage, pair="force")
     f = _state(k, shape, 3)
     d = k.desc(force=[2e-5, 0.0, 1e-5])
