@@ -78,8 +78,26 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
   const int b0 = bt * a.tb;
   const int nc = a.nc, na = a.na, nb = a.nb, P = a.pitch, w = a.w;
   const int tid = (int)threadIdx.x;
-  const int r = tid / w;
-  const int j = tid - r * w;
+  // (row r, column j) of the workgroup's rows; its LDS slot is r * w + j.
+  // Whole rows: thread tid is slot tid. C-tiles: the first R * tc threads are
+  // the inner columns, row by row, so each wave stores whole aligned lines;
+  // the next 2 R threads are the halo columns (j = 0 and w - 1 of each row).
+  int r, j;
+  if (a.halo) {
+    const int ninner = (a.tb + 2) * a.tc;
+    if (tid < ninner) {
+      r = tid / a.tc;
+      j = 1 + tid - r * a.tc;
+    } else {
+      const int k = tid - ninner;
+      r = k >> 1;
+      j = (k & 1) ? w - 1 : 0;
+    }
+  } else {
+    r = tid / w;
+    j = tid - r * w;
+  }
+  const int slot = r * w + j;
   // column before the periodic wrap: -1 .. ctiles * tc, below 2 * nc as nc >= tc
   const int cl = ct * a.tc - a.halo + j;
   const int c = cl < 0 ? cl + nc : (cl >= nc ? cl - nc : cl);
@@ -122,7 +140,7 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
     if (out && active) {
 #pragma unroll
       for (int q = 3; q < 19; ++q)
-        x[q - 3][tid] = L::ea[q] < 0 ? d[q] : (L::ea[q] == 0 ? Z[q] : P2[q]);
+        x[q - 3][slot] = L::ea[q] < 0 ? d[q] : (L::ea[q] == 0 ? Z[q] : P2[q]);
     }
     __syncthreads();
     if (p < a1 && active) fetch(p + 1, nx);
