@@ -503,8 +503,8 @@ constexpr bool pair_storage() {
 constexpr long long kPairCUs = 256;  // compute units of one MI355X
 // Rows of up to this many cells are held whole by a workgroup (at least
 // three of them); longer rows are cut into tiles of kPairTile columns plus
-// one halo column each side. 64 is the width that beat one step per launch
-// on every long-row lattice measured (32, 62, 126 and 128 each lost on one).
+// one halo column each side. 64 beat one step per launch by 1.16-1.39x on
+// every long-row lattice measured; 128 was about even with it, 32 slower.
 constexpr long long kPairWholeRows = 341;
 constexpr int kPairTile = 64;
 
@@ -513,18 +513,18 @@ constexpr int kPairTile = 64;
 // the call passes a force or tau field), no pressure faces, and 19 planes
 // addressable with 32-bit offsets. Rows of at most 341 cells are held whole
 // (three or more per 1024-thread workgroup); longer rows are cut into
-// 64-column tiles. Tiling rows that fit whole measured slower (256^3 f32dev,
-// MLUPS: whole rows 54.5k, tiles of 32 / 64 / 128 columns 41.1k / 43.5k /
-// 39.5k). The chunk length minimises rounds of one-workgroup-per-CU
+// 64-column tiles. Tiling rows that fit whole measured slower (f32dev
+// MLUPS, whole rows against 64-column tiles: 256^3 54.5k against 49.1k,
+// 192^3 50.1k against 40.0k). The chunk length minimises rounds of one-workgroup-per-CU
 // launches x (chunk + 2) planes.
 // Without LBM_FLAG_PAIR_FORCE it is used only where that measured faster
 // than two fused launches: at least 2^21 cells, at least 64 cells along a
 // and b (thinner axes recompute their halo planes and rows several times
 // over), and rows either of 64 to 256 cells in multiples of 32 (whole rows
 // of 288-341 cells leave a workgroup three rows, one of them its own
-// output) or longer than 341 cells (tiled: 128x128x512 39.6k against 36.5k
-// MLUPS, 256x256x512 39.2k against 34.9k, 64x64x2048 40.1k against 37.1k,
-// 96x96x1000 38.5k against 35.8k). Measured whole rows, f32dev, ms
+// output) or longer than 341 cells (tiled: 128x128x1024 49.1k against
+// 36.9k MLUPS, 256x256x512 48.1k against 34.7k, 128x128x512 43.5k against
+// 37.5k, 64x64x2048 44.3k against 37.3k, 96x96x1000 42.2k against 35.7k). Measured whole rows, f32dev, ms
 // per step, two-step against the best one-step launch shape: 256^3 0.31
 // against 0.45, 512x256x128 0.338 against 0.473, 320x320x192 0.440 against
 // 0.574, 192^3 0.154 against 0.203, 256x256x160 0.210 against 0.302, 160^3
