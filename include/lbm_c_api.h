@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LBM_API_VERSION 7
+#define LBM_API_VERSION 8
 
 typedef enum lbm_status {
   LBM_OK = 0,

@@ -10,6 +10,7 @@ import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -44,28 +45,55 @@ def library_path() -> str:
     return os.path.join(LIBDIR, LIBNAME)
 
 
-def build(verbose: bool = False) -> str:
-    """Compile the library (always recompiles) and return its path."""
-    os.makedirs(LIBDIR, exist_ok=True)
-    sources = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    if not sources:
-        raise RuntimeError(f"no HIP sources under {CSRC}")
-    out = library_path()
-    tmp = out + ".tmp"
-    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{os.path.join(os.path.dirname(HERE), 'include')}",
-           *sources, "-o", tmp, f"-Wl,-rpath,{rocm}/lib"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+def _compile(cmd: list[str]) -> tuple[int, str]:
     res = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass reports the device-only target feature as unknown
     noise = ("packed-fp32-ops",)
     msgs = [ln for ln in (res.stdout + res.stderr).splitlines()
             if ln.strip() and not any(n in ln for n in noise)]
-    if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + "\n".join(msgs))
-    if verbose and msgs:
-        print("\n".join(msgs))
+    return res.returncode, "\n".join(msgs)
+
+
+def build(verbose: bool = False, csrc: str = CSRC, out: str | None = None) -> str:
+    """Compile the library (always recompiles) and return its path.
+
+    Each translation unit compiles to an object in parallel (they share no
+    device code), then one link makes the shared library. ``csrc`` and
+    ``out`` name another source folder and output path (default: this
+    package's)."""
+    sources = sorted(glob.glob(os.path.join(csrc, "*.hip")))
+    if not sources:
+        raise RuntimeError(f"no HIP sources under {csrc}")
+    out = out or library_path()
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    objdir = os.path.join(os.path.dirname(out), "obj")
+    os.makedirs(objdir, exist_ok=True)
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    inc = f"-I{os.path.join(os.path.dirname(HERE), 'include')}"
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    objs, cmds = [], []
+    for src in sources:
+        obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+        objs.append(obj)
+        cmds.append([hipcc(), *compile_flags, inc, "-c", src, "-o", obj])
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        results = list(pool.map(_compile, cmds))
+    for cmd, (rc, msg) in zip(cmds, results):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+            if msg:
+                print(msg)
+        if rc != 0:
+            raise RuntimeError("hipcc failed:\n" + msg)
+    tmp = out + ".tmp"
+    link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
+            f"-Wl,-rpath,{rocm}/lib"]
+    rc, msg = _compile(link)
+    if verbose:
+        print(" ".join(link), flush=True)
+    if rc != 0:
+        raise RuntimeError("link failed:\n" + msg)
     os.replace(tmp, out)
     return out
 

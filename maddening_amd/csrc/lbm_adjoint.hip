@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       read_force<L, C>(a, n.cell, Fa, Fb, Fc);
       C drho, rho, ua, ub, uc;
       moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
-      const C om = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
+      const C om = a.tau_field ? recip(C(a.tau_field[n.cell])) : a.inv_tau;
       const C c = C(1) - C(0.5) * om;
       const C usq = ua * ua + ub * ub + uc * uc;
       const C uF = ua * Fa + ub * Fb + uc * Fc;
@@ -258,8 +258,8 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
         const C w = weight<L, C>(q);
         const C gq = gb[q] * w, gm = gb[m] * w;
         const C sm = gq + gm, df = gq - gm;
-        const C eu = C(L::ea[q]) * ua + C(L::eb[q]) * ub + C(L::ec[q]) * uc;
-        const C eF = C(L::ea[q]) * Fa + C(L::eb[q]) * Fb + C(L::ec[q]) * Fc;
+        const C eu = edot<L, C>(q, ua, ub, uc);
+        const C eF = edot<L, C>(q, Fa, Fb, Fc);
         G0 += sm;
         Geu = fmadd(df, eu, Geu);
         Geu2 = fmadd(sm * eu, eu, Geu2);
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       const C ubar_b = orho * (C(3) * Gb + C(9) * Kb - C(3) * ub * G0) + c * (C(9) * Hb - C(3) * Fb * G0);
       const C ubar_c = orho * (C(3) * Gc + C(9) * Kc - C(3) * uc * G0) + c * (C(9) * Hc - C(3) * Fc * G0);
       // u = (j + F/2) / rho: du/df_p = (e_p - u)/rho, drho/df_p = 1
-      const C inv_rho = C(1) / (rho > C(1e-10) ? rho : C(1e-10));
+      const C inv_rho = recip(rho > C(1e-10) ? rho : C(1e-10));
       const C Ba = ubar_a * inv_rho, Bb = ubar_b * inv_rho, Bc = ubar_c * inv_rho;
       const C A = rho_bar - (Ba * ua + Bb * ub + Bc * uc);
       const C keep = C(1) - om;
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
         C o[L::Q];
 #pragma unroll
         for (int p = 0; p < L::Q; ++p)
-          o[p] = keep * gb[p] + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
+          o[p] = keep * gb[p] + A + edot<L, C>(p, Ba, Bb, Bc);
         C rbf[2] = {C(0), C(0)};
         apply_faces_transpose<L, C>(a, ia, ib, ic, o, rbf);
         rb0 = solid ? 0.0 : double(rbf[0]);
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs<typename Storage<T>::C>
       } else {
 #pragma unroll
         for (int p = 0; p < L::Q; ++p) {
-          const C eB = C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc;
+          const C eB = edot<L, C>(p, Ba, Bb, Bc);
           const C g = xs ? gb[L::opp[p]] : gb[p];
           __builtin_nontemporal_store(solid ? C(0) : keep * g + A + (xs ? -eB : eB),
                                       a.gout + p * P + n.cell);
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(256) k_moments_adjoint(AdjArgs<typename Storag
         vc = C(a.u_bar[2 * cell + 1]);
       }
     }
-    const C inv_rho = C(1) / (rho > C(1e-10) ? rho : C(1e-10));
+    const C inv_rho = recip(rho > C(1e-10) ? rho : C(1e-10));
     const C Ba = va * inv_rho, Bb = vb * inv_rho, Bc = vc * inv_rho;
     const C A = rb - (Ba * ua + Bb * ub + Bc * uc);
     // at a wall cell u_bar is dropped (B = 0), so exchanging slots touches
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(256) k_moments_adjoint(AdjArgs<typename Storag
 #pragma unroll
       for (int p = 0; p < L::Q; ++p) {
         const C base = a.gin ? a.gin[p * P + cell] : C(0);
-        o[p] = base + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc);
+        o[p] = base + A + edot<L, C>(p, Ba, Bb, Bc);
       }
       const int ic = (int)(cell % a.nc);
       const int row = (int)(cell / a.nc);
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(256) k_moments_adjoint(AdjArgs<typename Storag
 #pragma unroll
       for (int p = 0; p < L::Q; ++p) {
         const C base = a.gin ? a.gin[(cross ? L::opp[p] : p) * P + cell] : C(0);
-        __builtin_nontemporal_store(base + A + (C(L::ea[p]) * Ba + C(L::eb[p]) * Bb + C(L::ec[p]) * Bc),
+        __builtin_nontemporal_store(base + A + edot<L, C>(p, Ba, Bb, Bc),
                                     a.gout + p * P + cell);
       }
     }

@@ -189,7 +189,7 @@ __device__ __forceinline__ void write_moments(const FwdArgs<C>& a, int cell,
 template <class L, class C>
 __device__ __forceinline__ C lid_term(const FwdArgs<C>& a, int q) {
   return C(6) * weight<L, C>(q) *
-         (C(L::ea[q]) * a.lid_a + C(L::eb[q]) * a.lid_b + C(L::ec[q]) * a.lid_c);
+         edot<L, C>(q, a.lid_a, a.lid_b, a.lid_c);
 }
 
 // The populations a cell holds after streaming and bounce-back, read from a
@@ -233,7 +233,7 @@ __device__ __forceinline__ void gather_state(const T* src, const uint8_t* mask, 
         // the population it reflects into direction q (wall density 1)
         if (m == 2)
           d[q] += C(6) * weight<L, C>(q) *
-                  (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c);
+                  edot<L, C>(q, lid_a, lid_b, lid_c);
       }
     } else {
       d[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
@@ -275,7 +275,7 @@ __device__ __forceinline__ void gather_links(const T* src, const uint32_t* links
     for (int q = 1; q < L::Q; ++q)
       if ((mv >> q) & 1u)
         d[q] += C(6) * weight<L, C>(q) *
-                (C(L::ea[q]) * lid_a + C(L::eb[q]) * lid_b + C(L::ec[q]) * lid_c);
+                edot<L, C>(q, lid_a, lid_b, lid_c);
   }
 }
 
@@ -311,7 +311,7 @@ __device__ __forceinline__ int cell_compute(const FwdArgs<typename Storage<T>::C
   moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
   if (a.rho || a.u || a.p) write_moments<L, C>(a, n.cell, self != 0, rho, ua, ub, uc);
   if (COLLIDE) {
-    const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
+    const C inv_tau = a.tau_field ? recip(C(a.tau_field[n.cell])) : a.inv_tau;
     collide_for<L, T, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
   }
   return n.cell;

@@ -133,6 +133,48 @@ template <> struct Storage<__bf16> {
   static __device__ __forceinline__ void store(__bf16* p, int q, float d) { put(p, enc<L>(q, d)); }
 };
 
+// e_q . (a, b, c) for a lattice vector (components -1, 0 or +1), optionally
+// added to `init`. Terms with a zero component are left out and +-1
+// products are exact, so the result equals init + e_a a + e_b b + e_c c
+// evaluated left to right for finite operands, without the multiplies by
+// zero and additions of zero that IEEE arithmetic would otherwise keep.
+template <class C>
+__device__ __forceinline__ void eterm(int e, C v, C& s, bool& any) {
+  if (e == 0) return;
+  const C t = e > 0 ? v : -v;
+  s = any ? s + t : t;
+  any = true;
+}
+template <class L, class C>
+__device__ __forceinline__ C edot(int q, C a, C b, C c) {
+  C s = C(0);
+  bool any = false;
+  eterm(L::ea[q], a, s, any);
+  eterm(L::eb[q], b, s, any);
+  eterm(L::ec[q], c, s, any);
+  return s;
+}
+template <class L, class C>
+__device__ __forceinline__ C edot_from(int q, C init, C a, C b, C c) {
+  C s = init;
+  bool any = true;
+  eterm(L::ea[q], a, s, any);
+  eterm(L::eb[q], b, s, any);
+  eterm(L::ec[q], c, s, any);
+  return s;
+}
+
+// 1 / x: v_rcp_f32 (within 1 ulp, x normal) for float, correctly rounded for
+// double. Every kernel takes these reciprocals here, so all round alike.
+template <class C>
+__device__ __forceinline__ C recip(C x) {
+  return C(1) / x;
+}
+template <>
+__device__ __forceinline__ float recip<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+
 // ---------------------------------------------------------------------------
 // Macroscopic moments with the Guo half-force correction
 // (lbm.py:209-239): rho = sum f, u = (sum f e + F/2) / max(rho, 1e-10).
@@ -154,7 +196,7 @@ __device__ __forceinline__ void moments(const C (&d)[L::Q], C Fa, C Fb, C Fc,
   }
   drho = s;
   rho = C(1) + s;
-  const C inv = C(1) / (rho > C(1e-10) ? rho : C(1e-10));
+  const C inv = recip(rho > C(1e-10) ? rho : C(1e-10));
   ua = (ja + C(0.5) * Fa) * inv;
   ub = (jb + C(0.5) * Fb) * inv;
   uc = (jc + C(0.5) * Fc) * inv;
@@ -223,8 +265,8 @@ __device__ __forceinline__ void collide_impl(C (&d)[L::Q], C inv_tau, C Fa, C Fb
     if (L::opp[q] < q) continue;  // each pair once, from its first member
     const int m = L::opp[q];
     const C w = weight<L, C>(q);
-    const C eu = C(L::ea[q]) * ua + C(L::eb[q]) * ub + C(L::ec[q]) * uc;
-    const C eF = C(L::ea[q]) * Fa + C(L::eb[q]) * Fb + C(L::ec[q]) * Fc;
+    const C eu = edot<L, C>(q, ua, ub, uc);
+    const C eF = edot<L, C>(q, Fa, Fb, Fc);
     const C A = fmadd(rho45 * eu, eu, fmadd(-rho, usq15, drho));  // even part of (feq - w)/w
     const C B = rho3 * eu;                                         // odd part
     const C sw = src * w;
@@ -255,8 +297,7 @@ __device__ __forceinline__ void collide_impl(C (&d)[L::Q], C inv_tau, C Fa, C Fb
     const C bc = C(3) * ((jc + Fc) - mc);
 #pragma unroll
     for (int q = 0; q < L::Q; ++q)
-      d[q] = fmadd(weight<L, C>(q), a0 + C(L::ea[q]) * ba + C(L::eb[q]) * bb + C(L::ec[q]) * bc,
-                   d[q]);
+      d[q] = fmadd(weight<L, C>(q), edot_from<L, C>(q, a0, ba, bb, bc), d[q]);
   }
 }
 

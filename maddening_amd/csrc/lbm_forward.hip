@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(256) k_push(FwdArgs<typename Storage<T>::C> a)
   load_force<L, C>(a, n.cell, Fa, Fb, Fc);
   C drho, rho, ua, ub, uc;
   moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
-  const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[n.cell]) : a.inv_tau;
+  const C inv_tau = a.tau_field ? recip(C(a.tau_field[n.cell])) : a.inv_tau;
   collide_for<L, T, C>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
@@ -213,7 +213,7 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
     moments<L, C>(dk, Fa, Fb, Fc, drho, rho, ua, ub, uc);
     if (a.rho || a.u || a.p) write_moments<L, C>(a, cell, false, rho, ua, ub, uc);
     if (COLLIDE) {
-      const C inv_tau = a.tau_field ? C(1) / C(a.tau_field[cell]) : a.inv_tau;
+      const C inv_tau = a.tau_field ? recip(C(a.tau_field[cell])) : a.inv_tau;
       collide_for<L, T, C>(dk, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
     }
 #pragma unroll
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(256)
   T* dst = static_cast<T*>(a.dst);
 #pragma unroll
   for (int q = 0; q < L::Q; ++q) {
-    const C eu = C(L::ea[q]) * ua + C(L::eb[q]) * ub + C(L::ec[q]) * uc;
+    const C eu = edot<L, C>(q, ua, ub, uc);
     const C dq = weight<L, C>(q) * (drho + rho * (C(3) * eu + C(4.5) * eu * eu - usq15));
     St::template store<L>(dst + q * a.plane + cell, q, dq);
   }
@@ -493,11 +493,18 @@ int launch_push(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>:
 // Storage types the two-step launches are built for: 32-bit storage, and
 // for D2Q9 16-bit deviation storage too. D3Q19 with 16-bit storage measured
 // slower than the row kernel (256^3: 569 us per two steps against 2 x 215
-// us), because k_pair is bound by load latency rather than by bytes.
+// us), because k_pair is bound by load latency rather than by bytes; there
+// only a single update in one launch (pair_one) uses the march.
 template <class L, class T>
 constexpr bool pair_storage() {
   return std::is_same<T, float>::value || std::is_same<T, f32dev_t>::value ||
          (L::Q == 9 && (std::is_same<T, _Float16>::value || std::is_same<T, __bf16>::value));
+}
+// Storage types of the one-launch update (k_pair<T, PAIR_LOCAL, PAIR_MOMENTS>).
+template <class L, class T>
+constexpr bool one_storage() {
+  return L::Q == 19 && (std::is_same<T, float>::value || std::is_same<T, f32dev_t>::value ||
+                        std::is_same<T, _Float16>::value || std::is_same<T, __bf16>::value);
 }
 
 constexpr long long kPairCUs = 256;  // compute units of one MI355X
@@ -508,40 +515,16 @@ constexpr long long kPairCUs = 256;  // compute units of one MI355X
 constexpr long long kPairWholeRows = 341;
 constexpr int kPairTile = 64;
 
-// Plan of the two-step launch for lattice g, or false where it does not
-// apply: D3Q19, periodic, 32-bit storage, uniform force and tau (`fields`:
-// the call passes a force or tau field), no pressure faces, and 19 planes
-// addressable with 32-bit offsets. Rows of at most 341 cells are held whole
-// (three or more per 1024-thread workgroup); longer rows are cut into
-// 64-column tiles. Tiling rows that fit whole measured slower (f32dev
-// MLUPS, whole rows against 64-column tiles: 256^3 54.5k against 49.1k,
-// 192^3 50.1k against 40.0k). The chunk length minimises rounds of one-workgroup-per-CU
-// launches x (chunk + 2) planes.
-// Without LBM_FLAG_PAIR_FORCE it is used only where that measured faster
-// than two fused launches: at least 2^21 cells, at least 64 cells along a
-// and b (thinner axes recompute their halo planes and rows several times
-// over), and rows either of 64 to 256 cells in multiples of 32 (whole rows
-// of 288-341 cells leave a workgroup three rows, one of them its own
-// output) or longer than 341 cells (tiled: 128x128x1024 49.1k against
-// 36.9k MLUPS, 256x256x512 48.1k against 34.7k, 128x128x512 43.5k against
-// 37.5k, 64x64x2048 44.3k against 37.3k, 96x96x1000 42.2k against 35.7k). Measured whole rows, f32dev, ms
-// per step, two-step against the best one-step launch shape: 256^3 0.31
-// against 0.45, 512x256x128 0.338 against 0.473, 320x320x192 0.440 against
-// 0.574, 192^3 0.154 against 0.203, 256x256x160 0.210 against 0.302, 160^3
-// 0.100 against 0.125, 128^3 0.056 against 0.066, 64x256x128 0.056 against
-// 0.065, 256x128x64 0.057 against 0.0645; below that size or with other row
-// lengths it broke even or lost (200x200x100 0.117 against 0.115, 100^3
-// 0.050 against 0.031, 96^3 0.045 against 0.028).
-inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
-  if (d->flags & LBM_FLAG_PAIR_OFF) return false;
-  const bool force = (d->flags & LBM_FLAG_PAIR_FORCE) != 0;
-  if (d->q != 19 || d->bb_mode != LBM_BB_NONE || fields) return false;
-  if (d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
-  if (d->inlet_face > 0 || d->outlet_face > 0) return false;
+// Geometry of the D3Q19 two-step march over lattice g, or false where it
+// cannot run (19 planes not addressable with 32-bit offsets). Rows of at most
+// 341 cells are held whole (three or more per 1024-thread workgroup); longer
+// rows are cut into 64-column tiles. Tiling rows that fit whole measured
+// slower (f32dev MLUPS, whole rows against 64-column tiles: 256^3 54.5k
+// against 49.1k, 192^3 50.1k against 40.0k). The chunk length minimises
+// rounds of one-workgroup-per-CU launches x (chunk + 2) planes.
+inline bool pair_geometry(const lbm_desc* d, const Geom& g, PairArgs* pa) {
+  if (d->q != 19) return false;
   if (19LL * g.plane >= (1LL << 31)) return false;
-  if (!force && (g.plane < (1LL << 21) || g.na < 64 || g.nb < 64 || g.nc < 64 ||
-                 (g.nc <= kPairWholeRows && (g.nc > 256 || g.nc % 32 != 0))))
-    return false;
   // whole rows up to kPairWholeRows cells, else c-tiles of kPairTile columns
   const bool tiled = g.nc > kPairWholeRows && g.nc >= kPairTile;
   const int w = tiled ? kPairTile + 2 : (int)g.nc;
@@ -588,6 +571,55 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   pa->Fb = (float)d->force[1];
   pa->Fc = (float)d->force[2];
   return true;
+}
+
+// Whether the two-step march can run this call at all: D3Q19, the reference
+// wall rule or none (not halfway bounce-back), and a lattice the geometry
+// covers.
+inline bool pair_applies(const lbm_desc* d, const Geom& g, PairArgs* pa) {
+  if (d->q != 19 || d->bb_mode == LBM_BB_HALFWAY) return false;
+  return pair_geometry(d, g, pa);
+}
+
+// Plan of the two-step launches of an lbm_step sequence over lattice g, or
+// false where lbm_step keeps one step per launch: 32-bit storage, the march
+// applies, and (without LBM_FLAG_PAIR_FORCE) the lattice is one where it
+// measured faster than two fused launches: periodic with uniform force and
+// tau (`fields`: the call passes a force or tau field) and no pressure faces,
+// at least 2^21 cells, at least 64 cells along a and b (thinner axes
+// recompute their halo planes and rows several times over), and rows either
+// of 64 to 256 cells in multiples of 32 (whole rows of 288-341 cells leave a
+// workgroup three rows, one of them its own output) or longer than 341 cells
+// (tiled: 128x128x1024 49.1k against 36.9k MLUPS, 256x256x512 48.1k against
+// 34.7k, 128x128x512 43.5k against 37.5k, 64x64x2048 44.3k against 37.3k,
+// 96x96x1000 42.2k against 35.7k). Measured whole rows, f32dev, ms per step,
+// two-step against the best one-step launch shape: 256^3 0.31 against 0.45,
+// 512x256x128 0.338 against 0.473, 320x320x192 0.440 against 0.574, 192^3
+// 0.154 against 0.203, 256x256x160 0.210 against 0.302, 160^3 0.100 against
+// 0.125, 128^3 0.056 against 0.066, 64x256x128 0.056 against 0.065,
+// 256x128x64 0.057 against 0.0645; below that size or with other row
+// lengths it broke even or lost (200x200x100 0.117 against 0.115, 100^3
+// 0.050 against 0.031, 96^3 0.045 against 0.028). With walls, fields or
+// faces the march runs its general form, which only PAIR_FORCE selects.
+inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
+  if (d->flags & LBM_FLAG_PAIR_OFF) return false;
+  if (d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
+  if (!pair_applies(d, g, pa)) return false;
+  if (d->flags & LBM_FLAG_PAIR_FORCE) return true;
+  if (d->bb_mode != LBM_BB_NONE || fields || d->inlet_face > 0 || d->outlet_face > 0) return false;
+  return !(g.plane < (1LL << 21) || g.na < 64 || g.nb < 64 || g.nc < 64 ||
+           (g.nc <= kPairWholeRows && (g.nc > 256 || g.nc % 32 != 0)));
+}
+
+// A single update (nsteps = 1) as one launch of the march, step 1 colliding
+// the stored state in place of a gather and step 2 writing the streamed
+// state and its moments (k_pair<T, PAIR_LOCAL, PAIR_MOMENTS>), in place of a
+// push launch plus a moments pass: D3Q19, 32- or 16-bit storage, no halfway
+// bounce-back, unless LBM_FLAG_PAIR_OFF.
+inline bool pair_one(const lbm_desc* d, const Geom& g, PairArgs* pa) {
+  if (d->flags & LBM_FLAG_PAIR_OFF) return false;
+  if (d->storage == LBM_F64) return false;
+  return pair_applies(d, g, pa);
 }
 
 // D2Q9 plan: periodic, 32-bit or 16-bit deviation storage (16-bit only
@@ -642,7 +674,8 @@ template <> struct PairOf<Lattice<19>> {
   static bool plan(const lbm_desc* d, const Geom& g, bool f, Args* a) { return pair_plan(d, g, f, a); }
   template <class T>
   static int launch(hipStream_t s, const Args& pa) {
-    hipLaunchKernelGGL((k_pair<T>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s, pa);
+    hipLaunchKernelGGL((k_pair<T, PAIR_GATHER, PAIR_COLLIDE, false>), dim3((unsigned)pa.nblocks),
+                       dim3(1024), 0, s, pa);
     return check_launch("k_pair launch");
   }
 };
@@ -656,24 +689,82 @@ template <> struct PairOf<Lattice<9>> {
   }
 };
 
-// Launches of a pull sequence of nsteps steps: collide, npair two-step
-// launches, nsingle fused launches, stream + moments.
+// One two-step launch of kind (S1, S2): the general form where the call has
+// walls, fields or faces, the periodic form otherwise (fewer registers).
+template <class T, int S1, int S2>
+int launch_pair_gen(hipStream_t s, const PairArgs& pa) {
+  if (pa.mask || pa.force_field || pa.tau_field || pa.zh_n)
+    hipLaunchKernelGGL((k_pair<T, S1, S2, true>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s,
+                       pa);
+  else
+    hipLaunchKernelGGL((k_pair<T, S1, S2, false>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s,
+                       pa);
+  return check_launch("k_pair launch");
+}
+
+// The general fields of a two-step launch from the single-step arguments of
+// the same call: mask (reference wall rule only), force and tau fields,
+// faces.
+template <class C>
+void pair_general(PairArgs* pa, const FwdArgs<C>& a, bool bb_ref) {
+  pa->mask = bb_ref ? a.mask : nullptr;
+  pa->force_field = a.force_field;
+  pa->tau_field = a.tau_field;
+  pa->zh_n = a.zh_n;
+  for (int i = 0; i < 2; ++i) {
+    pa->zh_axis[i] = a.zh_axis[i];
+    pa->zh_side[i] = a.zh_side[i];
+    pa->zh_rho[i] = float(a.zh_rho[i]);
+  }
+}
+
+// Launches of a pull sequence of nsteps steps. Its units are one collision
+// C, then nsteps - 1 fused steps SC, then one stream + moments S.
+//  * One step per launch: C, SC x (nsteps - 1), S.
+//  * D2Q9 two-step launches: C, pairs of SC, an odd SC, S.
+//  * D3Q19 two-step launches (`fold`): every launch but at most one is a
+//    two-step launch. nsteps = 1: (C, S) in one launch; nsteps = 2: C, then
+//    (SC, S); nsteps >= 3: (C, SC), the odd SC when nsteps is even, pairs of
+//    SC, (SC, S).
 struct PullSchedule {
-  int npair, nsingle, launches;
+  int launches;   // all launches
+  int pairs;      // two-step launches among them
+  int npair;      // (SC, SC) launches
+  int nsingle;    // single SC launches
+  bool fold;      // the first and last units run inside two-step launches
 };
-inline PullSchedule pull_schedule(int nsteps, bool pair) {
+inline PullSchedule pull_schedule(int nsteps, bool pair, bool fold) {
   PullSchedule sc;
-  const int fused = nsteps - 1;
-  sc.npair = pair ? fused / 2 : 0;
-  sc.nsingle = fused - 2 * sc.npair;
-  sc.launches = 2 + sc.npair + sc.nsingle;
+  sc.fold = pair && fold;
+  if (!sc.fold) {
+    const int fused = nsteps - 1;
+    sc.npair = pair ? fused / 2 : 0;
+    sc.nsingle = fused - 2 * sc.npair;
+    sc.launches = 2 + sc.npair + sc.nsingle;
+    sc.pairs = sc.npair;
+    return sc;
+  }
+  if (nsteps == 1) {
+    sc.npair = sc.nsingle = 0;
+    sc.launches = sc.pairs = 1;
+  } else if (nsteps == 2) {
+    sc.npair = sc.nsingle = 0;  // the lone collision runs single, then (SC, S)
+    sc.launches = 2;
+    sc.pairs = 1;
+  } else {
+    sc.nsingle = nsteps % 2 == 0 ? 1 : 0;
+    sc.npair = (nsteps - 3 - sc.nsingle) / 2;
+    sc.launches = 2 + sc.nsingle + sc.npair;
+    sc.pairs = 2 + sc.npair;
+  }
   return sc;
 }
 
-// Pull sequence: collide (f_in -> B0), fused pull+collide (B -> B'), two
-// fused steps per launch where `pair` is given, final pull + moments
-// (B -> f_out). Launch j writes buf[(launches - 1 - j) & 1], so the last
-// writes f_out and reads `work`.
+// Pull sequence (launch j writes buf[(launches - 1 - j) & 1], so the last
+// writes f_out and reads `work`; a one-launch sequence needs no workspace):
+// collide (f_in -> B0), fused pull+collide (B -> B'), final pull + moments
+// (B -> f_out), with two fused steps per launch where `pair` is given and,
+// for D3Q19, the first and last units folded into two-step launches.
 template <class L, class T, int BB>
 int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
               const void* f_in, void* f_out, void* work, int nsteps,
@@ -681,20 +772,66 @@ int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
               const typename PairOf<L>::Args* pair = nullptr) {
   using C = typename Storage<T>::C;
   int rc;
-  const PullSchedule sc = pull_schedule(nsteps, pair != nullptr);
+  constexpr bool kFold = L::Q == 19 && (BB == BB_NONE || BB == BB_REF);
+  const PullSchedule sc = pull_schedule(nsteps, pair != nullptr, kFold);
   void* buf[2] = {f_out, work};
   int j = 0;
+  auto next = [&]() { return buf[(sc.launches - 1 - j++) & 1]; };
   FwdArgs<C> k = a;
   k.rho = k.u = k.p = nullptr;
+  if constexpr (L::Q == 19 && (BB == BB_NONE || BB == BB_REF) && one_storage<L, T>()) {
+    if (sc.fold) {
+      PairArgs pa = *pair;
+      pair_general(&pa, a, BB == BB_REF);
+      if (sc.launches == 1) {
+        // (C, S): the whole update in one launch
+        pa.src = f_in;
+        pa.dst = next();
+        pa.rho = rho; pa.u = u; pa.p = p;
+        return launch_pair_gen<T, PAIR_LOCAL, PAIR_MOMENTS>(s, pa);
+      }
+      if constexpr (pair_storage<L, T>()) {
+        const void* cur = f_in;
+        if (nsteps == 2) {
+          k.src = f_in;
+          k.dst = next();
+          if ((rc = launch_cell<L, T, BB, false, true, true>(s, g, k))) return rc;
+          cur = k.dst;
+        } else {
+          pa.src = f_in;
+          pa.dst = next();
+          if ((rc = launch_pair_gen<T, PAIR_LOCAL, PAIR_COLLIDE>(s, pa))) return rc;
+          cur = pa.dst;
+          for (int i = 0; i < sc.nsingle; ++i) {
+            k.src = cur;
+            k.dst = next();
+            if ((rc = launch_cell<L, T, BB, true, true, true>(s, g, k))) return rc;
+            cur = k.dst;
+          }
+          for (int i = 0; i < sc.npair; ++i) {
+            pa.src = cur;
+            pa.dst = next();
+            rc = launch_pair_gen<T, PAIR_GATHER, PAIR_COLLIDE>(s, pa);
+            if (rc) return rc;
+            cur = pa.dst;
+          }
+        }
+        pa.src = cur;
+        pa.dst = next();
+        pa.rho = rho; pa.u = u; pa.p = p;
+        return launch_pair_gen<T, PAIR_GATHER, PAIR_MOMENTS>(s, pa);
+      }
+    }
+  }
   k.src = f_in;
-  k.dst = buf[(sc.launches - 1 - j++) & 1];
+  k.dst = next();
   if ((rc = launch_cell<L, T, BB, false, true, true>(s, g, k))) return rc;
   if constexpr (BB == BB_NONE && pair_storage<L, T>()) {
     if (sc.npair) {
       typename PairOf<L>::Args pa = *pair;
       for (int i = 0; i < sc.npair; ++i) {
         pa.src = k.dst;
-        pa.dst = buf[(sc.launches - 1 - j++) & 1];
+        pa.dst = next();
         if ((rc = PairOf<L>::template launch<T>(s, pa))) return rc;
         k.dst = pa.dst;
       }
@@ -702,7 +839,7 @@ int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
   }
   for (int i = 0; i < sc.nsingle; ++i) {
     k.src = k.dst;
-    k.dst = buf[(sc.launches - 1 - j++) & 1];
+    k.dst = next();
     if ((rc = launch_cell<L, T, BB, true, true, true>(s, g, k))) return rc;
   }
   k.src = work;
@@ -766,14 +903,23 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
   Geom gr = g;
   gr.rowv = scheme == LBM_SCHEME_PULL ? row_width(d, g) : 0;
   if (scheme == LBM_SCHEME_PULL) {
+    // two-step launches: a single update as one launch (pair_one), longer
+    // sequences where pair_plan measured them faster (or PAIR_FORCE)
+    typename PairOf<L>::Args pa;
+    bool pair = false;
+    if constexpr (L::Q == 19) {
+      if (nsteps == 1) pair = one_storage<L, T>() && pair_one(d, g, &pa);
+      else pair = pair_storage<L, T>() && pair_plan(d, g, ff || tf, &pa);
+    } else {
+      pair = pair_storage<L, T>() && PairOf<L>::plan(d, g, ff || tf, &pa);
+    }
     switch (d->bb_mode) {
-      case LBM_BB_NONE: {
-        typename PairOf<L>::Args pa;
-        const bool pair = pair_storage<L, T>() && PairOf<L>::plan(d, g, ff || tf, &pa);
+      case LBM_BB_NONE:
         return step_pull<L, T, BB_NONE>(s, gr, a, f_in, f_out, work, nsteps, rho, u, p,
                                         pair ? &pa : nullptr);
-      }
-      case LBM_BB_REFERENCE: return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p);
+      case LBM_BB_REFERENCE:
+        return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p,
+                                       pair ? &pa : nullptr);
       default: {
         uint32_t* links = reinterpret_cast<uint32_t*>(static_cast<char*>(work) + links_offset(d, g));
         if (int rc = LinksOp<L, T>::run(s, g, mask, links)) return rc;
@@ -844,7 +990,7 @@ struct PullPhaseOp {
     if (phase == 2) {
       a.rho = rho; a.u = u; a.p = p;
     }
-    if (phase == 3) {
+    if (phase == 3 && L::Q == 9) {
       typename PairOf<L>::Args pa;
       if (!pair_storage<L, T>() || !PairOf<L>::plan(d, g, ff || tf, &pa))
         return fail(LBM_ERR_UNSUPPORTED,
@@ -854,6 +1000,29 @@ struct PullPhaseOp {
         pa.dst = dst;
         return PairOf<L>::template launch<T>(s, pa);
       }
+    }
+    if (phase >= 3) {
+      // D3Q19 two-step launches: 3 (SC, SC), 4 (C, SC), 5 (SC, S), 6 (C, S)
+      PairArgs pa;
+      const bool ok = phase == 6 ? one_storage<L, T>() : pair_storage<L, T>();
+      if (!ok || !pair_applies(d, g, &pa))
+        return fail(LBM_ERR_UNSUPPORTED, "phase " + std::to_string(phase) +
+                                             " (two steps per launch) does not apply to this lattice");
+      if constexpr (one_storage<L, T>()) {
+        pair_general(&pa, a, d->bb_mode == LBM_BB_REFERENCE);
+        pa.src = src;
+        pa.dst = dst;
+        if (phase >= 5) {
+          pa.rho = rho; pa.u = u; pa.p = p;
+        }
+        if (phase == 6) return launch_pair_gen<T, PAIR_LOCAL, PAIR_MOMENTS>(s, pa);
+        if constexpr (pair_storage<L, T>()) {
+          if (phase == 3) return launch_pair_gen<T, PAIR_GATHER, PAIR_COLLIDE>(s, pa);
+          if (phase == 4) return launch_pair_gen<T, PAIR_LOCAL, PAIR_COLLIDE>(s, pa);
+          return launch_pair_gen<T, PAIR_GATHER, PAIR_MOMENTS>(s, pa);
+        }
+      }
+      return fail(LBM_ERR_UNSUPPORTED, "two steps per launch do not apply to this lattice");
     }
     Geom gr = g;
     gr.rowv = row_width(d, g);
@@ -946,20 +1115,40 @@ size_t lbm_state_bytes(const lbm_desc* d) {
   return (size_t)g.plane * (size_t)d->q * (size_t)storage_size(d->storage);
 }
 
-// Auto: one push launch for a single step (pull needs two); the pull
-// sequence for longer runs, measured 15-20 % faster per step than push.
-static int resolve_scheme(const lbm_desc* d, int nsteps) {
-  if (d->scheme != LBM_SCHEME_AUTO) return d->scheme;
-  return nsteps <= 1 ? LBM_SCHEME_PUSH : LBM_SCHEME_PULL;
+// The scheme and launches of an lbm_step call. Auto: the pull sequence,
+// measured 15-20 % faster per step than push, and for a single D3Q19 step
+// its one-launch form (pair_one); push for a single step elsewhere (the
+// pull sequence needs two launches, push one plus the moments pass).
+struct CallPlan {
+  int scheme;
+  bool pair;
+  PullSchedule sc;
+};
+static CallPlan call_plan(const lbm_desc* d, const Geom& g, int nsteps, bool fields) {
+  CallPlan cp;
+  PairArgs pa;
+  Pair9Args pa9;
+  const bool one = d->q == 19 && nsteps == 1 && pair_one(d, g, &pa);
+  cp.scheme = d->scheme;
+  if (cp.scheme == LBM_SCHEME_AUTO)
+    cp.scheme = (nsteps <= 1 && !one) ? LBM_SCHEME_PUSH : LBM_SCHEME_PULL;
+  cp.pair = false;
+  if (cp.scheme == LBM_SCHEME_PULL) {
+    if (d->q == 19) cp.pair = nsteps == 1 ? one : pair_plan(d, g, fields, &pa);
+    else cp.pair = pair9_plan(d, g, fields, &pa9);
+  }
+  cp.sc = pull_schedule(nsteps, cp.pair, d->q == 19 && d->bb_mode != LBM_BB_HALFWAY);
+  return cp;
 }
 
 size_t lbm_workspace_size(const lbm_desc* d, int nsteps) {
   Geom g;
   if (check_desc(d, &g)) return 0;
-  const int scheme = resolve_scheme(d, nsteps);
-  if (scheme == LBM_SCHEME_PUSH && nsteps <= 1) return 0;
-  if (scheme == LBM_SCHEME_PULL && d->bb_mode == LBM_BB_HALFWAY)
+  const CallPlan cp = call_plan(d, g, nsteps < 1 ? 1 : nsteps, false);
+  if (cp.scheme == LBM_SCHEME_PUSH && nsteps <= 1) return 0;
+  if (cp.scheme == LBM_SCHEME_PULL && d->bb_mode == LBM_BB_HALFWAY)
     return links_offset(d, g) + 2 * (size_t)g.plane * sizeof(uint32_t);
+  if (cp.scheme == LBM_SCHEME_PULL && cp.sc.launches == 1) return 0;
   return (size_t)g.plane * (size_t)d->q * (size_t)storage_size(d->storage);
 }
 
@@ -976,7 +1165,7 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out, voi
   if (f_in == f_out) return fail(LBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
   if (d->bb_mode != LBM_BB_NONE && !mask)
     return fail(LBM_ERR_INVALID, "bb_mode needs a wall mask");
-  const int scheme = resolve_scheme(d, nsteps);
+  const int scheme = call_plan(d, g, nsteps, force_field || tau_field).scheme;
   if (lbm_workspace_size(d, nsteps) > 0 && !work)
     return fail(LBM_ERR_INVALID, "this step needs a workspace of lbm_workspace_size() bytes");
   if (work && (work == f_in || work == f_out))
@@ -993,16 +1182,12 @@ int lbm_step_schedule(const lbm_desc* d, int nsteps, int with_fields, int* launc
   if (rc) return rc;
   if (nsteps < 1) return fail(LBM_ERR_INVALID, "nsteps must be >= 1");
   int nl, np = 0;
-  if (resolve_scheme(d, nsteps) == LBM_SCHEME_PUSH) {
+  const CallPlan cp = call_plan(d, g, nsteps, with_fields != 0);
+  if (cp.scheme == LBM_SCHEME_PUSH) {
     nl = nsteps + 1;  // one launch per step, then the moments pass
   } else {
-    PairArgs pa;
-    Pair9Args pa9;
-    const bool pair = d->q == 19 ? pair_plan(d, g, with_fields != 0, &pa)
-                                 : pair9_plan(d, g, with_fields != 0, &pa9);
-    const PullSchedule sc = pull_schedule(nsteps, pair);
-    nl = sc.launches;
-    np = sc.npair;
+    nl = cp.sc.launches;
+    np = cp.sc.pairs;
   }
   if (launches) *launches = nl;
   if (pair_launches) *pair_launches = np;
@@ -1017,7 +1202,7 @@ int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, 
   Geom g;
   int rc = check_desc(d, &g);
   if (rc) return rc;
-  if (phase < 0 || phase > 3) return fail(LBM_ERR_INVALID, "phase must be 0, 1, 2 or 3");
+  if (phase < 0 || phase > 6) return fail(LBM_ERR_INVALID, "phase must be 0 to 6");
   if (!src || !dst) return fail(LBM_ERR_INVALID, "src and dst must be non-NULL");
   if (src == dst) return fail(LBM_ERR_INVALID, "src and dst must be distinct buffers");
   if (d->bb_mode != LBM_BB_NONE && !mask)

@@ -39,6 +39,7 @@
 #include <hip/hip_runtime.h>
 
 #include "lbm_common.hpp"
+#include "lbm_cell.hpp"
 
 namespace mlbm {
 
@@ -59,6 +60,52 @@ struct PairArgs {
   int nblocks;     // ctiles * btiles * ceil(na / la)
   int remap;       // xcd_remap run length (0: identity)
   float inv_tau, Fa, Fb, Fc;
+  // Read by the general launches (GEN) only: the reference wall rule at
+  // mask != 0, per-cell force (*G, 3) and tau (*G) fields, Zou-He faces;
+  // each pointer nullable. rho/u/p: the moments a PAIR_MOMENTS launch writes.
+  const uint8_t* mask;
+  const float* force_field;
+  const float* tau_field;
+  float* rho;
+  float* u;
+  float* p;
+  int zh_n;
+  int zh_axis[2];  // internal axis 0 (a), 1 (b), 2 (c)
+  int zh_side[2];  // 0: first layer, 1: last layer
+  float zh_rho[2];
+};
+
+// What the two halves of a launch do. Step 1 reads either the post-collision
+// state g_{n-1} at x - e_q (PAIR_GATHER: streaming, then the wall rule and
+// faces) or the pre-collision state f_n at x (PAIR_LOCAL: an aligned read,
+// the first launch of a sequence), and collides. Step 2 gathers from step
+// 1's results, applies the wall rule and faces, and either collides and
+// stores g (PAIR_COLLIDE) or stores the streamed state f with its moments
+// (PAIR_MOMENTS: the last launch of a sequence, or, after PAIR_LOCAL, one
+// whole LBMNode.update in one launch).
+enum { PAIR_GATHER = 0, PAIR_LOCAL = 1 };
+enum { PAIR_COLLIDE = 0, PAIR_MOMENTS = 1 };
+
+// Reference wall rule (lbm.py:944-948) after a gather: a wall cell's slot q
+// takes the value streamed into slot opp(q), i.e. the two slots of every
+// opposite pair are exchanged.
+template <class L, class C>
+__device__ __forceinline__ void swap_pairs(C (&d)[L::Q]) {
+#pragma unroll
+  for (int q = 1; q < L::Q; ++q) {
+    if (L::opp[q] < q) continue;
+    const C t = d[q];
+    d[q] = d[L::opp[q]];
+    d[L::opp[q]] = t;
+  }
+}
+
+// Per-cell inputs of a general launch fetched with the cell's populations
+// and carried to step 2. A force field is read where it is used instead
+// (three more values per carried plane would spill registers).
+struct PairAux {
+  uint32_t m;  // mask byte (0: fluid)
+  float tau;
 };
 
 // The march of one workgroup over c-tile ct, b-tile bt and a-chunk `chunk`
@@ -66,7 +113,7 @@ struct PairArgs {
 // (w == nc) the periodic wrap along c stays inside the workgroup; with
 // c-tiles the halo columns compute step 1 only, and step 2 of the tc inner
 // columns reads them.
-template <class T>
+template <class T, int S1, int S2, bool GEN>
 __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, int chunk,
                                            float (&x)[16][1024]) {
   using L = Lattice<19>;
@@ -108,28 +155,56 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
   const T* src = static_cast<const T*>(a.src);
   T* dst = static_cast<T*>(a.dst);
 
-  auto fetch = [&](int p, C (&v)[19]) {
+  // step 1's input at plane p: the populations, and for a general launch the
+  // cell's mask byte, force and tau (loaded with them, used after they land)
+  auto fetch = [&](int p, C (&v)[19], PairAux& au) {
     Nbr n;
     n.init((p + na) % na, b, c, na, nb, nc);
 #pragma unroll
-    for (int q = 0; q < 19; ++q)
-      v[q] = St::template load<L>(src + q * P + n.template at<L>(q, -1), q);
+    for (int q = 0; q < 19; ++q) {
+      const int at = S1 == PAIR_LOCAL ? n.cell : n.template at<L>(q, -1);
+      v[q] = St::template load<L>(src + q * P + at, q);
+    }
+    if constexpr (GEN) {
+      au.m = a.mask ? a.mask[n.cell] : 0u;
+      au.tau = a.tau_field ? a.tau_field[n.cell] : 0.0f;
+    }
+  };
+  auto inv_tau_of = [&](const PairAux& au) {
+    return GEN && a.tau_field ? recip(au.tau) : a.inv_tau;
+  };
+  auto force_at = [&](int cell, C& Fa, C& Fb, C& Fc) {
+    Fa = a.Fa;
+    Fb = a.Fb;
+    Fc = a.Fc;
+    if (GEN && a.force_field) {
+      Fa = a.force_field[3 * cell + 0];
+      Fb = a.force_field[3 * cell + 1];
+      Fc = a.force_field[3 * cell + 2];
+    }
   };
 
   // own column's step-1 values: Z of plane p - 1 (e_a = 0), P1 / P2 of planes
   // p - 1 / p - 2 (e_a = +1); only the entries of those classes are used
   C Z[19], P1[19], P2[19], nx[19];
+  PairAux nau = {0u, 0.0f}, cur = nau, prev = nau;
 #pragma unroll
   for (int q = 0; q < 19; ++q) Z[q] = P1[q] = P2[q] = C(0);
-  if (active) fetch(a0 - 1, nx);
+  if (active) fetch(a0 - 1, nx, nau);
   for (int p = a0 - 1; p <= a1; ++p) {
     C d[19];
 #pragma unroll
     for (int q = 0; q < 19; ++q) d[q] = nx[q];
+    if constexpr (GEN) cur = nau;
     {
-      C drho, rho, ua, ub, uc;
-      moments<L, C>(d, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
-      collide_for<L, T, C>(d, a.inv_tau, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      if constexpr (GEN && S1 == PAIR_GATHER) {
+        if (cur.m) swap_pairs<L, C>(d);
+        else if (a.zh_n) apply_faces<L, C>(a, (p + na) % na, b, c, d);
+      }
+      C drho, rho, ua, ub, uc, Fa, Fb, Fc;
+      force_at((((p + na) % na) * nb + b) * nc + c, Fa, Fb, Fc);
+      moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C>(d, inv_tau_of(cur), Fa, Fb, Fc, drho, rho, ua, ub, uc);
 #pragma unroll
       for (int q = 0; q < 19; ++q) {  // the value a store and reload would give
         const T s = St::template enc<L>(q, d[q]);
@@ -143,7 +218,7 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
         x[q - 3][slot] = L::ea[q] < 0 ? d[q] : (L::ea[q] == 0 ? Z[q] : P2[q]);
     }
     __syncthreads();
-    if (p < a1 && active) fetch(p + 1, nx);
+    if (p < a1 && active) fetch(p + 1, nx, nau);
     if (out && inner) {
       C e[19];
       e[0] = Z[0];   // (p - 1, b, c)
@@ -157,14 +232,32 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
         cc = cc < 0 ? cc + w : (cc >= w ? cc - w : cc);
         e[q] = x[q - 3][(r - L::eb[q]) * w + cc];
       }
-      C drho, rho, ua, ub, uc;
-      moments<L, C>(e, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
-      collide_for<L, T, C>(e, a.inv_tau, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      if constexpr (GEN) {
+        if (prev.m) swap_pairs<L, C>(e);
+        else if (a.zh_n) apply_faces<L, C>(a, p - 1, b, c, e);
+      }
       const int cell = ((p - 1) * nb + b) * nc + c;
+      C drho, rho, ua, ub, uc, Fa, Fb, Fc;
+      force_at(cell, Fa, Fb, Fc);
+      moments<L, C>(e, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      if constexpr (S2 == PAIR_MOMENTS) {
+        // the stream + moments launch's outputs (lbm.py:970-976)
+        if (a.rho) a.rho[cell] = rho;
+        if (a.p) a.p[cell] = rho * C(1.0 / 3.0);
+        if (a.u) {
+          const bool wall = GEN && prev.m;
+          a.u[3 * cell + 0] = wall ? C(0) : ua;
+          a.u[3 * cell + 1] = wall ? C(0) : ub;
+          a.u[3 * cell + 2] = wall ? C(0) : uc;
+        }
+      } else {
+        collide_for<L, T, C>(e, inv_tau_of(prev), Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      }
 #pragma unroll
       for (int q = 0; q < 19; ++q) St::template store<L>(dst + q * P + cell, q, e[q]);
     }
     __syncthreads();
+    if constexpr (GEN) prev = cur;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
       if (L::ea[q] > 0) {
@@ -269,12 +362,12 @@ __device__ __forceinline__ void pair_march_ring(const PairArgs& a, int bt, int c
   }
 }
 
-template <class T>
+template <class T, int S1, int S2, bool GEN>
 __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   __shared__ float x[16][1024];
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   const int rest = lb / a.ctiles;
-  pair_march<T>(a, lb - rest * a.ctiles, rest % a.btiles, rest / a.btiles, x);
+  pair_march<T, S1, S2, GEN>(a, lb - rest * a.ctiles, rest % a.btiles, rest / a.btiles, x);
 }
 
 // ---------------------------------------------------------------------------

@@ -70,7 +70,7 @@ __device__ __forceinline__ T w19(int q) {
 template <class T>
 __device__ __forceinline__ T feq(int q, T rho, T ua, T ub, T uc, T usq) {
   using L = Lattice<19>;
-  const T eu = T(L::ea[q]) * ua + T(L::eb[q]) * ub + T(L::ec[q]) * uc;
+  const T eu = edot<L, T>(q, ua, ub, uc);
   return w19<T>(q) * rho * (T(1) + T(3) * eu + T(4.5) * eu * eu - T(1.5) * usq);
 }
 
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(256) k_pipe(PipeArgs<T> a) {
       // g_eq = w7 C (1 + e.u/cs2); no collision at walls (lbm_pipe.py:256-271, 838-843)
 #pragma unroll
       for (int q = 0; q < 7; ++q) {
-        const T eu = T(Q7::ea[q]) * oa + T(Q7::eb[q]) * ob + T(Q7::ec[q]) * oc;
+        const T eu = edot<Q7, T>(q, oa, ob, oc);
         const T ge = T(Q7::w8[q]) / T(8) * c * (T(1) + T(3) * eu);
         g[q] = g[q] - (g[q] - ge) / a.tau_tracer;
       }
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_pipe(PipeArgs<T> a) {
   const T pref = T(1) - T(0.5) / a.tau;
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const T eu = T(L::ea[q]) * ua + T(L::eb[q]) * ub + T(L::ec[q]) * uc;
+    const T eu = edot<L, T>(q, ua, ub, uc);
     const T eF = T(L::ea[q]) * Fa + T(L::ec[q]) * Fc;
     const T fe = feq<T>(q, rho, ua, ub, uc, usq);
     const T S = pref * w19<T>(q) * (T(3) * (eF - uF) + T(9) * eu * eF);

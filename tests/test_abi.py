@@ -43,7 +43,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_string_without_gpu():
     from maddening_amd import _lib
     L = _lib.lib()
-    assert L.lbm_version() == 7
+    assert L.lbm_version() == 8
     assert isinstance(L.lbm_last_error(), bytes)
 
 

@@ -284,6 +284,48 @@ def test_zou_he_faces_f32_periodic_row_path_off():
     assert rel(g["density"], o["density"]) < 1e-5
 
 
+@pytest.mark.parametrize("scheme,pair", [("pull", "off"), ("pull", "auto"), ("push", "auto"),
+                                         ("auto", "auto")])
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("lattice,axis", [("D3Q19", 0), ("D3Q19", 1), ("D3Q19", 2),
+                                          ("D2Q9", 0), ("D2Q9", 1)])
+def test_zou_he_resting_face_closed_form(lattice, axis, storage, scheme, pair):
+    """The reference's face rule (lbm.py:476-479, 503) in closed form: a
+    lattice at rest at rho_p with both faces of one axis at rho_p. After one
+    step every unknown slot of a fluid face cell is exactly 2 w_q rho_p (the
+    textbook Zou-He rule would leave w_q rho_p), a wall cell on the face and
+    every bulk cell keep w rho_p. Covers the pull sequence's stream + moments
+    launch (pair off), the push path's face kernel and the one-launch
+    update (D3Q19 float32 pull or auto)."""
+    from maddening_amd.lbm import LBMKernel
+    Q = 19 if lattice == "D3Q19" else 9
+    e, w, _ = orc.lattice(Q)
+    shape = (6, 5, 8) if Q == 19 else (6, 8)
+    D = len(shape)
+    rho_p = 1.02
+    f0 = np.broadcast_to(w * rho_p, shape + (Q,)).copy()
+    mask = np.zeros(shape, np.uint8)
+    idx = [2] * D
+    idx[axis] = 0
+    mask[tuple(idx)] = 1
+    names = LBMKernel.FACES[2 * axis: 2 * axis + 2]
+    k = kernel(shape, lattice, 0.8, storage=storage, scheme=scheme, bb="reference", pair=pair)
+    f = k.from_aos(torch.from_numpy(f0).cuda())
+    fo, _, _, _ = k.step(f, 1, mask=torch.from_numpy(mask).cuda(),
+                         faces=((names[0], rho_p), (names[1], rho_p)))
+    got = k.to_aos(fo, torch.float64).cpu().numpy()
+    expect = f0.copy()
+    for side, x in (("min", 0), ("max", shape[axis] - 1)):
+        _, unknown, _ = orc.classify_directions(e, axis, side)
+        sl = [slice(None)] * D
+        sl[axis] = x
+        for q in unknown:
+            expect[tuple(sl) + (q,)] = np.where(mask[tuple(sl)] != 0, w[q] * rho_p,
+                                                2 * w[q] * rho_p)
+    tol = 1e-14 if storage == "f64" else 2e-6
+    np.testing.assert_allclose(got, expect, rtol=tol, atol=0)
+
+
 # degenerate and ragged extents: axes of length 1 and 2 (periodic neighbours
 # wrap onto the cell itself or onto the same partner from both sides), rows
 # shorter than a wave and not a multiple of 64, prime extents

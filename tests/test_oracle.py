@@ -292,3 +292,132 @@ def test_oracle_float32_mode_stays_float32():
     st = orc.initial_state((4, 4, 4), 19, dtype=np.float32)
     out = orc.update(st["f"], tau=0.8, force=(1e-5, 0.0, 0.0))
     assert all(out[k].dtype == np.float32 for k in ("f", "density", "velocity", "pressure"))
+
+
+# --- Zou-He pressure face: closed forms of the reference's rule ---------------
+# (lbm.py:425-510). Its normal velocity is u_n = +-(1 - (sum known + sum
+# tangential) / rho_p), without the textbook factor 2 on the known sum
+# (lbm.py:476-479), applied through f_q = f_opp + feq_q - feq_opp at (rho_p,
+# u_n) (:503). Two consequences fix that rule and fail the textbook one:
+#  * on a resting face f = w rho_p the unknown weights sum to 1/6 (both
+#    lattices), so u_n = +-1/6 and every unknown slot becomes exactly
+#    2 w_q rho_p (the textbook u_n is 0 there and leaves w_q rho_p);
+#  * for any populations the face cell's density afterwards is
+#    rho_p + sum known (the textbook rule gives rho_p).
+FACES_OF = {19: ["x_min", "x_max", "y_min", "y_max", "z_min", "z_max"],
+            9: ["x_min", "x_max", "y_min", "y_max"]}
+ZH_CASES = [(Q, face) for Q in (19, 9) for face in FACES_OF[Q]]
+
+
+def _face_setup(Q, face):
+    e, w, opp = orc.lattice(Q)
+    D = e.shape[1]
+    shape = (5, 4, 6) if D == 3 else (5, 6)
+    axis, side = orc.FACES[face]
+    sl = [slice(None)] * D
+    sl[axis] = 0 if side == "min" else shape[axis] - 1
+    wall = np.zeros(shape, bool)
+    idx = [1] * D
+    idx[axis] = sl[axis]
+    wall[tuple(idx)] = True                     # one wall cell on the face
+    wall[(2,) * D] = True                       # and one off it
+    return e, w, opp, shape, axis, side, tuple(sl), wall
+
+
+def _check_resting_face(zh, Q, face, rho_p=1.03):
+    e, w, opp, shape, axis, side, sl, wall = _face_setup(Q, face)
+    known, unknown, tang = orc.classify_directions(e, axis, side)
+    assert w[unknown].sum() == pytest.approx(1 / 6, abs=1e-16)
+    f = np.broadcast_to(w * rho_p, shape + (Q,)).copy()
+    out = zh(f, rho_p, e, w, face, wall)
+    expect = f.copy()
+    fluid = ~wall[sl]
+    for q in unknown:
+        expect[sl + (q,)] = np.where(fluid, 2.0 * w[q] * rho_p, w[q] * rho_p)
+    np.testing.assert_allclose(out, expect, rtol=1e-14, atol=0)
+
+
+def _check_face_density(zh, Q, face, rho_p=0.97):
+    e, w, opp, shape, axis, side, sl, wall = _face_setup(Q, face)
+    known, unknown, tang = orc.classify_directions(e, axis, side)
+    rng = np.random.default_rng(Q + len(face))
+    f = w * (1 + 0.2 * rng.standard_normal(shape + (Q,)))
+    out = zh(f, rho_p, e, w, face, wall)
+    fluid = ~wall[sl]
+    rho_new = out[sl].sum(-1)
+    expect = rho_p + f[sl][..., known].sum(-1)
+    np.testing.assert_allclose(rho_new[fluid], expect[fluid], rtol=1e-13)
+    # wall face cells and every cell off the face keep their populations
+    np.testing.assert_array_equal(out[sl][~fluid], f[sl][~fluid])
+    off = np.ones(shape, bool)
+    off[sl] = False
+    np.testing.assert_array_equal(out[off], f[off])
+    # known and tangential slots are never written
+    np.testing.assert_array_equal(out[..., known + tang], f[..., known + tang])
+
+
+@pytest.mark.parametrize("Q,face", ZH_CASES)
+def test_zou_he_resting_face_doubles_unknown_populations(Q, face):
+    _check_resting_face(orc.zou_he_pressure_face, Q, face)
+
+
+@pytest.mark.parametrize("Q,face", ZH_CASES)
+def test_zou_he_face_density_is_rho_p_plus_known(Q, face):
+    _check_face_density(orc.zou_he_pressure_face, Q, face)
+
+
+def _textbook_zou_he(f, rho_p, e, w, face, wall):
+    """A planted mistake: the textbook Zou-He normal velocity
+    u_n = +-(1 - (sum tangential + 2 sum known) / rho_p), otherwise the same
+    non-equilibrium bounce-back."""
+    axis, side = orc.FACES[face]
+    D = e.shape[1]
+    known, unknown, tang = orc.classify_directions(e, axis, side)
+    sl = [slice(None)] * D
+    sl[axis] = 0 if side == "min" else -1
+    sl = tuple(sl)
+    ff = f[sl]
+    s = ff[..., tang].sum(-1) + 2.0 * ff[..., known].sum(-1)
+    un = (1.0 - s / rho_p) if side == "min" else (-1.0 + s / rho_p)
+    new = ff.copy()
+    for q in unknown:
+        p = int(orc._opposites(e)[q])
+        cand = ff[..., p] + 6.0 * w[q] * rho_p * e[q, axis] * un
+        new[..., q] = np.where(wall[sl], ff[..., q], cand)
+    out = f.copy()
+    out[sl] = new
+    return out
+
+
+@pytest.mark.parametrize("Q,face", [(19, "x_min"), (19, "z_max"), (9, "y_max")])
+def test_zou_he_pins_reject_the_textbook_rule(Q, face):
+    """Both closed forms fail on the textbook normal velocity, so neither
+    can pass on a restatement that slipped back to it."""
+    with pytest.raises(AssertionError):
+        _check_resting_face(_textbook_zou_he, Q, face)
+    with pytest.raises(AssertionError):
+        _check_face_density(_textbook_zou_he, Q, face)
+
+
+@pytest.mark.parametrize("Q", [19, 9])
+def test_zou_he_full_update_on_resting_lattice(Q):
+    """Through the whole update (lbm.py:899-986): a lattice at rest at
+    rho_p with an inlet and an outlet face at that density, one step. Bulk
+    cells keep w rho_p; fluid cells of both faces end with 2 w_q rho_p in
+    their unknown slots; a wall cell on the inlet face is only bounced,
+    which leaves a resting cell unchanged."""
+    e, w, opp = orc.lattice(Q)
+    D = e.shape[1]
+    shape = (6, 5, 4) if D == 3 else (6, 5)
+    rho_p = 1.02
+    f = np.broadcast_to(w * rho_p, shape + (Q,)).copy()
+    mask = np.zeros(shape, np.uint8)
+    mask[(0,) + (2,) * (D - 1)] = 1
+    out = orc.update(f, tau=0.8, mask=mask, inlet=("x_min", rho_p * CS2),
+                     outlet=("x_max", rho_p * CS2))["f"]
+    expect = f.copy()
+    for side, x in (("min", 0), ("max", shape[0] - 1)):
+        _, unknown, _ = orc.classify_directions(e, 0, side)
+        for q in unknown:
+            expect[x][..., q] = np.where(mask[x] != 0, w[q] * rho_p, 2 * w[q] * rho_p)
+    np.testing.assert_allclose(out, expect, rtol=1e-14, atol=0)

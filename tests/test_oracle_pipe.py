@@ -149,3 +149,96 @@ def test_multiphase_floor_and_tracer_from_density():
     assert np.array_equal(out["tracer_f"], st["tracer_f"])          # frozen
     t = out["tracer"]
     assert t.min() >= 0 and t.max() <= 1 and np.all(t[wall] == 0)
+
+
+# --- which velocity each collision uses (lbm_pipe.py:781-787, 833-858) ------
+# On a wall-free periodic box in a uniform state at rest, streaming moves
+# nothing, so one step of the fluid is exactly its collision. The reference
+# collides with the RAW velocity sum f e / rho (velocity_eq = velocity_raw,
+# :783) in both the equilibrium and the Guo source, so from rest
+#   f_q' = w_q (rho + 3 (1 - 1/(2 tau)) e_q . F)        (S_q at u = 0),
+# and the total momentum grows by exactly (1 - 1/(2 tau)) F per cell and step
+# (u + F/2 in the equilibrium would make that F). The tracer is collided
+# with the step's OUTPUT velocity (:858), which is (1 - 1/(2 tau)) F / rho
+# after one step from rest, so its flux after one step is
+#   sum_q g_q' e_q = 0.75 C u_new / tau_tracer
+# (the pre-step velocity would give 0).
+def _uniform_box(n=(4, 3, 5), rho=1.0, c=0.7):
+    wall = np.zeros(n, bool)
+    prop = np.ones(n, bool)                      # propeller force on every cell
+    u = np.zeros(n + (3,))
+    st = {"f": op.equilibrium(np.full(n, rho), u), "density": np.full(n, rho), "velocity": u,
+          "tracer": np.full(n, c), "tracer_f": op.tracer_equilibrium(np.full(n, c), u)}
+    return wall, prop, st
+
+
+def _check_raw_velocity_collision(update, tau=0.8, Fx=2e-3, gz=-1e-3, rho=1.0):
+    wall, prop, st = _uniform_box(rho=rho)
+    out = update(st, tau=tau, tau_tracer=0.6, wall=wall, prop=prop, prop_strength=Fx,
+                 gravity=gz)
+    F = np.array([Fx, 0.0, gz])
+    src = 1.0 - 0.5 / tau
+    expect = op.W * (rho + 3.0 * src * (op.E @ F))
+    np.testing.assert_allclose(out["f"], np.broadcast_to(expect, out["f"].shape),
+                               rtol=1e-14, atol=0)
+    st3 = op.run(st, 3, tau=tau, tau_tracer=0.6, wall=wall, prop=prop, prop_strength=Fx,
+                 gravity=gz) if update is op.update else None
+    if st3 is not None:
+        mom = (st3["f"] @ op.E).sum(axis=(0, 1, 2))
+        np.testing.assert_allclose(mom, 3 * src * F * wall.size, rtol=1e-12, atol=1e-13)
+
+
+def _check_tracer_velocity(update, tau=0.8, tau_t=0.6, Fx=2e-3, gz=-1e-3, rho=1.0, c=0.7):
+    wall, prop, st = _uniform_box(rho=rho, c=c)
+    out = update(st, tau=tau, tau_tracer=tau_t, wall=wall, prop=prop, prop_strength=Fx,
+                 gravity=gz)
+    u_new = (1.0 - 0.5 / tau) * np.array([Fx, 0.0, gz]) / rho
+    np.testing.assert_allclose(out["velocity"], np.broadcast_to(u_new, out["velocity"].shape),
+                               rtol=1e-13, atol=1e-16)
+    flux = out["tracer_f"] @ op.E7
+    np.testing.assert_allclose(flux, np.broadcast_to(0.75 * c * u_new / tau_t, flux.shape),
+                               rtol=1e-12, atol=1e-19)
+
+
+def test_pipe_collision_uses_raw_velocity():
+    _check_raw_velocity_collision(op.update)
+
+
+def test_pipe_tracer_collides_with_post_step_velocity():
+    _check_tracer_velocity(op.update)
+
+
+def test_pipe_velocity_pins_reject_the_alternatives(monkeypatch):
+    """Planted mistakes in the oracle, each of which one of the two pins
+    above must catch: u + F/2 in the fluid equilibrium, u + F/2 in the Guo
+    source only, and the tracer collided with the pre-step velocity."""
+    orig_eq, orig_guo, orig_teq = op.equilibrium, op.guo_forcing, op.tracer_equilibrium
+    # the force of _check_raw_velocity_collision, per cell (rho = 1)
+    half = 0.5 * np.array([2e-3, 0.0, -1e-3])
+
+    def shifted_update(patch):
+        def upd(st, **kw):
+            with monkeypatch.context() as m:
+                patch(m)
+                return op.update(st, **kw)
+        return upd
+
+    def eq_shift(m):
+        calls = {"n": 0}
+
+        def eq(rho, u):
+            calls["n"] += 1
+            return orig_eq(rho, u + half if calls["n"] == 1 else u)
+        m.setattr(op, "equilibrium", eq)
+
+    def guo_shift(m):
+        m.setattr(op, "guo_forcing", lambda u, F, tau: orig_guo(u + half, F, tau))
+
+    def tracer_pre(m):
+        m.setattr(op, "tracer_equilibrium", lambda c, u: orig_teq(c, np.zeros_like(u)))
+
+    for patch in (eq_shift, guo_shift):
+        with pytest.raises(AssertionError):
+            _check_raw_velocity_collision(shifted_update(patch))
+    with pytest.raises(AssertionError):
+        _check_tracer_velocity(shifted_update(tracer_pre))

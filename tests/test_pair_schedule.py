@@ -1,5 +1,10 @@
 """Which lbm_step calls run two steps per launch (LBM_FLAG_PAIR_*), checked on
-the CPU through lbm_step_schedule, which makes no HIP call."""
+the CPU through lbm_step_schedule, which makes no HIP call.
+
+A D3Q19 pull sequence of n steps is n + 1 units (a collision, n - 1 fused
+steps, a stream + moments); with two-step launches every launch but at most
+one carries two units, so the call makes (n + 2) // 2 launches, (n + 1) // 2
+of them two-step launches, and a single update is one launch."""
 import ctypes
 
 import pytest
@@ -25,12 +30,14 @@ def _schedule(d, nsteps, with_fields=0):
     return nl.value, npair.value
 
 
+def _folded(nsteps):
+    return (nsteps + 2) // 2, (nsteps + 1) // 2
+
+
 @pytest.mark.parametrize("storage", [0, 4])  # f32, f32dev
-@pytest.mark.parametrize("nsteps", [2, 3, 1000, 1001])
+@pytest.mark.parametrize("nsteps", [1, 2, 3, 4, 20, 1000, 1001])
 def test_auto_pairs_at_256_cubed(storage, nsteps):
-    nl, npair = _schedule(_desc((256, 256, 256), storage), nsteps)
-    assert npair == (nsteps - 1) // 2
-    assert nl == 2 + npair + (nsteps - 1) % 2
+    assert _schedule(_desc((256, 256, 256), storage), nsteps) == _folded(nsteps)
 
 
 # lattices where two-step launches measured faster than the best one-step
@@ -43,7 +50,7 @@ def test_auto_pairs_at_256_cubed(storage, nsteps):
                                   (128, 128, 512), (256, 256, 512), (64, 64, 2048),
                                   (96, 96, 1000), (64, 96, 342)])
 def test_auto_pairs_on_measured_winning_shapes(dims):
-    assert _schedule(_desc(dims), 9) == (6, 4)
+    assert _schedule(_desc(dims), 9) == (5, 5)
 
 
 def test_auto_keeps_single_launches_where_pairs_measured_slower():
@@ -76,7 +83,13 @@ def test_force_flag_reaches_every_row_length():
     # whole rows up to 341 cells, 64-column tiles beyond
     for dims in [(1, 1, 64), (33, 50, 64), (128, 128, 128), (3, 2, 256), (8, 8, 96),
                  (2, 3, 341), (1, 1, 1), (8, 8, 342), (8, 8, 512), (2, 1, 2000)]:
-        assert _schedule(_desc(dims, flags=F), 9) == (6, 4)
+        assert _schedule(_desc(dims, flags=F), 9) == (5, 5)
+    # the general form: reference walls, fields, pressure faces
+    for d in [_desc((8, 8, 64), bb=1, flags=F), _desc((8, 8, 64), faces=True, flags=F)]:
+        assert _schedule(d, 9) == (5, 5)
+    assert _schedule(_desc((8, 8, 64), flags=F), 9, with_fields=1) == (5, 5)
+    # never under halfway bounce-back
+    assert _schedule(_desc((8, 8, 64), bb=2, flags=F), 9) == (10, 0)
     # not 16-bit or 64-bit storage
     for d in [_desc((8, 8, 64), storage=2, flags=F), _desc((8, 8, 512), storage=3, flags=F),
               _desc((8, 8, 64), storage=1, flags=F)]:
@@ -90,7 +103,18 @@ def test_phase_3_rejected_where_pairs_do_not_apply():
     rc = L.lbm_pull_phase(None, ctypes.byref(d), 3, ctypes.c_void_p(4096), ctypes.c_void_p(8192),
                           None, None, None, None, None, None, None)
     assert rc == -2 and b"two steps per launch" in L.lbm_last_error()
-    assert L.lbm_pull_phase(None, ctypes.byref(d), 4, ctypes.c_void_p(4096),
+    for phase in (3, 4, 5):   # sequences: 32-bit storage only
+        assert L.lbm_pull_phase(None, ctypes.byref(d), phase, ctypes.c_void_p(4096),
+                                ctypes.c_void_p(8192), None, None, None, None, None, None,
+                                None) == -2
+    d64 = _desc((8, 8, 64), storage=1)
+    assert L.lbm_pull_phase(None, ctypes.byref(d64), 6, ctypes.c_void_p(4096),
+                            ctypes.c_void_p(8192), None, None, None, None, None, None, None) == -2
+    dh = _desc((8, 8, 64), bb=2)
+    assert L.lbm_pull_phase(None, ctypes.byref(dh), 6, ctypes.c_void_p(4096),
+                            ctypes.c_void_p(8192), ctypes.c_void_p(64), None, None, None, None,
+                            None, None) == -2
+    assert L.lbm_pull_phase(None, ctypes.byref(d), 7, ctypes.c_void_p(4096),
                             ctypes.c_void_p(8192), None, None, None, None, None, None, None) == -1
 
 
@@ -115,3 +139,32 @@ def test_d2q9_16_bit_pairs_only_when_forced():
         assert _schedule(_desc((64, 100), storage=storage, q=9, flags=F), 9) == (6, 4)
     # never float64
     assert _schedule(_desc((64, 100), storage=1, q=9, flags=F), 9) == (10, 0)
+
+
+def test_single_update_is_one_launch():
+    """nsteps = 1 (LBMNode.update's call): one two-step launch that collides
+    the stored state and writes the streamed state with its moments, for
+    D3Q19 with 32- or 16-bit storage, periodic or the reference wall rule,
+    with or without fields and faces, any lattice shape, and no workspace.
+    Float64, halfway bounce-back, D2Q9 and PAIR_OFF keep the push launch plus
+    the moments pass."""
+    from maddening_amd import _lib
+    L = _lib.lib()
+    for storage in (0, 2, 3, 4):
+        for d in [_desc((256, 256, 256), storage), _desc((64, 32, 32), storage, bb=1),
+                  _desc((5, 3, 7), storage, faces=True), _desc((1, 1, 1), storage),
+                  _desc((8, 8, 1000), storage, bb=1)]:
+            assert _schedule(d, 1) == (1, 1)
+            assert _schedule(d, 1, with_fields=1) == (1, 1)
+            assert L.lbm_workspace_size(ctypes.byref(d), 1) == 0
+    for d in [_desc((64, 32, 32), storage=1), _desc((64, 32, 32), bb=2),
+              _desc((64, 32), q=9), _desc((64, 32, 32), flags=_lib.LBM_FLAG_PAIR_OFF)]:
+        assert _schedule(d, 1) == (2, 0)
+    # an explicit push scheme stays push
+    d = _desc((64, 32, 32))
+    d.scheme = _lib.LBM_SCHEME_PUSH
+    assert _schedule(d, 1) == (2, 0)
+    # pull with PAIR_OFF: collide, then stream + moments
+    d = _desc((64, 32, 32), flags=_lib.LBM_FLAG_PAIR_OFF)
+    d.scheme = _lib.LBM_SCHEME_PULL
+    assert _schedule(d, 1) == (2, 0)
