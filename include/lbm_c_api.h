@@ -239,6 +239,20 @@ size_t lbm_reduce_workspace_size(const lbm_desc* d);
 int lbm_conservation(void* stream, const lbm_desc* d, const void* f,
                      void* work, double* out);
 
+/*
+ * lbm_conservation with no workspace argument and no allocation: its
+ * per-workgroup partial sums live in a 32 KiB array the library holds on
+ * each GPU it runs on, until the process ends. No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It calls no hipMalloc, hipFree or allocating routine. The
+ * first call on a device, made with that device current, looks up the
+ * array's address. Every call on one GPU shares its array: calls that
+ * overlap on one GPU (on different streams) race, so order them on one
+ * stream. Same results as lbm_conservation, bit for bit.
+ */
+int lbm_conservation_static(void* stream, const lbm_desc* d, const void* f, double* out);
+
 /* ---- batches of independent lattices ------------------------------------ */
 /*
  * No counterpart in the reference: its only batching is jax.vmap over

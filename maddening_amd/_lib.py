@@ -131,6 +131,7 @@ _PROTOS = {
                                            _vp, _vp, _vp, _vp, _vp, _vp]),
     "lbm_adjoint_end": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),
     "lbm_conservation": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp, _vp]),
+    "lbm_conservation_static": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),
     "lbm_adjoint_face_end": (ctypes.c_int, [_vp, ctypes.POINTER(LbmDesc), _vp, _vp]),
     "lbm_pipe_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(LbmPipeDesc)]),
     "lbm_pipe_step": (ctypes.c_int, [_vp, ctypes.POINTER(LbmPipeDesc), _vp, _vp, _vp, _vp, _vp,

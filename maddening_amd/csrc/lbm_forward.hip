@@ -19,6 +19,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -1103,6 +1104,41 @@ struct ConserveOp {
 
 using namespace mlbm;
 
+// Partial sums of lbm_conservation_static: 4 doubles per workgroup of
+// k_conserve_partial, at most kReduceBlocks workgroups (32 KiB). One copy per
+// GPU, alive until the process ends. Host code must not pass `scratch`
+// itself to a kernel (on the host it names a stand-in, not device memory):
+// it takes the device address from scratch_address().
+static __device__ __align__(256) double scratch[4 * kReduceBlocks];
+constexpr size_t kScratchBytes = sizeof(double) * 4 * kReduceBlocks;
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress) and kept.
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(LBM_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(LBM_ERR_UNSUPPORTED, "device ordinal " + std::to_string(dev) + " above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(LBM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                   hipGetErrorString(e));
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return LBM_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int lbm_version(void) { return LBM_API_VERSION; }
@@ -1266,6 +1302,20 @@ int lbm_convert(void* stream, const lbm_desc* d, int to_soa, int aos_is_f64, con
 
 size_t lbm_reduce_workspace_size(const lbm_desc*) {
   return sizeof(double) * 4 * kReduceBlocks;
+}
+
+int lbm_conservation_static(void* stream, const lbm_desc* d, const void* f, double* out) {
+  g_last_error.clear();
+  (void)hipGetLastError();
+  Geom g;
+  int rc = check_desc(d, &g);
+  if (rc) return rc;
+  if (!f || !out) return fail(LBM_ERR_INVALID, "f and out must be non-NULL");
+  if (lbm_reduce_workspace_size(d) > kScratchBytes)
+    return fail(LBM_ERR_UNSUPPORTED, "the reduction needs more than the 32 KiB static scratch");
+  void* work = nullptr;
+  if ((rc = scratch_address(&work))) return rc;
+  return dispatch<ConserveOp>(d, (hipStream_t)stream, d, g, f, work, out);
 }
 
 int lbm_conservation(void* stream, const lbm_desc* d, const void* f, void* work, double* out) {
