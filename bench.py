@@ -415,6 +415,149 @@ def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=16, reps=3):
     return res
 
 
+def time_phase(kern, phase, src, reps=20, mask=None, force_field=None, tau_field=None,
+               moments=False):
+    """Average duration of one lbm_pull_phase launch of `phase`, HIP events
+    on the stream the library launches on (torch's current stream)."""
+    import ctypes
+    import torch
+    from maddening_amd import _lib
+    from maddening_amd.lbm import _stream
+    d = kern.desc()
+    L = _lib.lib()
+    s = _stream(kern.device)
+    dst = kern.empty()
+    mom = [None, None, None]
+    if moments:
+        mom = [torch.empty(kern.grid_shape, device=kern.device),
+               torch.empty(kern.grid_shape + (kern.D,), device=kern.device),
+               torch.empty(kern.grid_shape, device=kern.device)]
+
+    def p(t):
+        return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+    def launch():
+        _lib.check(L.lbm_pull_phase(s, ctypes.byref(d), phase, p(src), p(dst), p(mask), None,
+                                    p(force_field), p(tau_field), *[p(t) for t in mom]),
+                   "lbm_pull_phase")
+    for _ in range(3):
+        launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def node_update_leg(dev, nsteps=50, reps=3):
+    """The drop-in call pattern: GraphManager.step, which calls
+    LBMNode.update once per step (graph_manager.py:2436-2438), float32 state
+    (the node's). 256^3 periodic Taylor-Green, and the 128x128x256 sphere
+    channel (reference walls, uniform body force (0, 0, 1e-5) as a boundary
+    input). Each update is one launch (k_pair<float, local collide, stream +
+    moments>); `pair off` is the same loop with a push launch plus a moments
+    pass per update. The one-launch kernel moves 152 B of state + 20 B of
+    moments per cell (+1 B of mask with walls)."""
+    import torch
+    from maddening_amd import workloads
+    from maddening_amd.graph import GraphManager
+    from maddening_amd.node import LBMNode
+    out = {}
+    n = 256
+    rho, u = taylor_green_device(n, 0.05, dev)
+    chan = (128, 128, 256)
+    mask_np = workloads.sphere_packing(chan, porosity=0.7, rmin=6, rmax=12, seed=0)
+    for name, shape, mask, ext in (
+            ("taylor_green_256", (n, n, n), None, {}),
+            ("sphere_channel_128x128x256", chan, mask_np, {"body_force": [0.0, 0.0, 1e-5]})):
+        cells = shape[0] * shape[1] * shape[2]
+        rec = {"grid": list(shape), "updates": nsteps}
+        for pair in ("auto", "off"):
+            node = LBMNode("fluid", 1.0, grid_shape=shape, viscosity=(0.8 - 0.5) / 3.0,
+                           wall_mask=mask, device=dev, pair=pair)
+            k = node.kernel
+            gm = GraphManager()
+            gm.add_node(node)
+            if mask is None:
+                st = gm.get_node_state("fluid")
+                st["f"] = k.aos_view(k.equilibrium(rho, u))
+                gm.set_node_state("fluid", st)
+            gm.compile()
+            e = {"fluid": dict(ext)}
+            gm.run(3, external_inputs=e)
+            torch.cuda.synchronize()
+            times = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                gm.run(nsteps, external_inputs=e)
+                e1.record()
+                torch.cuda.synchronize()
+                times.append(e0.elapsed_time(e1) / 1e3 / nsteps)
+            t = statistics.median(times)
+            r = {"ms_per_update": round(t * 1e3, 4), "MLUPS": round(cells / t / 1e6, 1),
+                 "launches_per_update": k.schedule(1)[0]}
+            if pair == "auto":
+                st = gm.get_node_state("fluid")
+                src = node._soa(st["f"])
+                m = st["wall_mask"].to(torch.uint8).contiguous()
+                kt = time_phase(k, 6, src, mask=m, moments=True)
+                bpc = 152 + 20 + (1 if mask is not None else 0)
+                r["kernel_us"] = round(kt * 1e6, 2)
+                r["kernel_GBs"] = round(bpc * cells / kt / 1e9, 1)
+                r["kernel_bytes_per_cell"] = bpc
+                del src, m, st
+            rec["one_launch" if pair == "auto" else "push_plus_moments"] = r
+            del node, gm, k
+            torch.cuda.empty_cache()
+        rec["speedup"] = round(rec["push_plus_moments"]["ms_per_update"]
+                               / rec["one_launch"]["ms_per_update"], 3)
+        out[name] = rec
+    return out
+
+
+def config4_forward_leg(dev, nsteps=500, shape=(128, 128, 256), reps=3):
+    """Forward of the sphere-packed channel (porosity 0.7, radii 6-12, seed 0;
+    reference walls, tau field 0.7, force (0, 0, 1e-5)), float32: one fused
+    launch per step against the general two-step launches (walls and the tau
+    field inside the LDS march); the same bits either way."""
+    import torch
+    from maddening_amd import workloads
+    from maddening_amd.lbm import LBMKernel
+    mask = torch.from_numpy(workloads.sphere_packing(shape, porosity=0.7, rmin=6, rmax=12,
+                                                     seed=0)).to(dev)
+    tau = torch.full(shape, 0.7, device=dev)
+    res = {}
+    outs = {}
+    for pair in ("auto", "force"):
+        k = LBMKernel(shape, tau=0.7, storage="f32", bb="reference", pair=pair, device=dev)
+        f0 = k.equilibrium()
+        work = k.workspace(nsteps)
+        o = k.empty()
+        k.step(f0, 10, mask=mask, force=[0, 0, 1e-5], tau_field=tau, work=work, out=o)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            k.step(f0, nsteps, mask=mask, force=[0, 0, 1e-5], tau_field=tau, work=work, out=o)
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1) / nsteps)
+        outs[pair] = o.clone()
+        res["single" if pair == "auto" else "two_step"] = {
+            "ms_per_step": round(statistics.median(times), 4),
+            "launches": k.schedule(nsteps, with_fields=True)[0]}
+        del k, f0, work, o
+        torch.cuda.empty_cache()
+    res["same_bits"] = bool(torch.equal(outs["auto"], outs["force"]))
+    res["speedup"] = round(res["single"]["ms_per_step"] / res["two_step"]["ms_per_step"], 3)
+    return res
+
+
 def batch_leg(dev, nsteps=200, seed=0):
     """64 lattices (32 D2Q9 edge U[64,1024], 32 D3Q19 edge U[32,160], tau
     U[0.55,1.5], Taylor-Green or lid-driven-cavity init), float32: one
@@ -644,12 +787,16 @@ def main():
                          "the default: same bytes as f32, 10x smaller rounding) or f32")
     ap.add_argument("--variants", default="f32,f16dev,bf16dev",
                     help="extra storage types to time (comma list, '' for none)")
-    ap.add_argument("--legs", default="precision,adjoint,batch,pipe,d2q9,long_rows",
+    ap.add_argument("--legs",
+                    default="precision,adjoint,node_update,config4_forward,batch,pipe,d2q9,long_rows",
                     help="extra measurements: precision (storage types vs float64 at "
                          "100/1000/5000 steps), adjoint (forward+adjoint ms/step, gradient "
                          "vs central differences), batch (64 uneven lattices), pipe "
                          "(LBMPipeNode single phase and multiphase), d2q9 (8192^2 D2Q9), "
-                         "long_rows (D3Q19 128x128x1024, rows in 64-column tiles)")
+                         "long_rows (D3Q19 128x128x1024, rows in 64-column tiles), "
+                         "node_update (GraphManager.step with LBMNode.update, one launch per "
+                         "update), config4_forward (the sphere channel one or two steps per "
+                         "launch)")
     ap.add_argument("--pair", default="auto", choices=["auto", "off", "force"],
                     help="two pull steps per launch (LBM_FLAG_PAIR_*)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -796,6 +943,10 @@ def main():
         result["precision"] = precision_leg(n, dev)
     if "adjoint" in legs:
         result["adjoint"] = adjoint_leg(dev)
+    if "node_update" in legs:
+        result["node_update"] = node_update_leg(dev)
+    if "config4_forward" in legs:
+        result["config4_forward"] = config4_forward_leg(dev)
     if "batch" in legs:
         result["batch"] = batch_leg(dev)
     if "pipe" in legs:

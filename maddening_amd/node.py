@@ -87,7 +87,9 @@ class LBMNode(SimulationNode):
 
     Same constructor as the reference (lbm.py:604-615) plus keyword-only
     build options: ``storage`` ("f32" or "f64"), ``bb`` ("reference", the
-    reference's wall rule, or "halfway"), ``device`` and ``scheme``.
+    reference's wall rule, or "halfway"), ``device``, ``scheme`` and
+    ``pair`` (two steps per launch, see ``LBMKernel``; with "auto" each
+    D3Q19 ``update`` is one launch).
     """
 
     def __init__(self, name: str, timestep: float, grid_shape: tuple = (64, 32, 32),
@@ -95,7 +97,7 @@ class LBMNode(SimulationNode):
                  wall_mask: Optional[np.ndarray] = None, inlet_face: str = "x_min",
                  outlet_face: str = "x_max", geometry_source: Optional[str] = None, *,
                  storage: str = "f32", bb: str = "reference", device="cuda",
-                 scheme: str = "auto"):
+                 scheme: str = "auto", pair: str = "auto"):
         if lattice.upper() == "D3Q19":
             lat = d3q19()
         elif lattice.upper() == "D2Q9":
@@ -131,7 +133,7 @@ class LBMNode(SimulationNode):
         self._inlet_face = inlet_face
         self._outlet_face = outlet_face
         self._kernel = LBMKernel(self._grid_shape, lattice=lat.name, tau=tau, storage=storage,
-                                 bb=bb, scheme=scheme, device=device)
+                                 bb=bb, scheme=scheme, pair=pair, device=device)
         self._bb = bb
         self._device = self._kernel.device
         self._dtype = self._kernel.dtype
