@@ -116,15 +116,18 @@ typedef struct lbm_desc {
  * except the one that returns dL/df_0, and on lbm_moments_adjoint. */
 #define LBM_FLAG_ADJ_OUT_FACES 8
 /* Two steps per launch (no counterpart in the reference: a launch schedule
- * of the same update). lbm_step runs pairs of its fused pull steps as one
- * launch that keeps the intermediate state in LDS, where the lattice allows
- * it: LBM_BB_NONE, LBM_F32 or LBM_F32_DEV (D2Q9 under PAIR_FORCE also
- * LBM_F16_DEV and LBM_BF16_DEV), uniform force and tau, no pressure faces,
- * Q * cells < 2^31. By default only where it measured faster (D3Q19: at
- * least 2^21 cells, dims[0] and dims[1] at least 64, dims[2] one of 64, 96,
- * ..., 256 or above 341; D2Q9: at least 2048^2 cells); PAIR_FORCE uses it
- * wherever it applies, PAIR_OFF never. Results are bit-identical either
- * way. */
+ * of the same update). lbm_step runs its pull sequence two units per launch
+ * (a collision, the fused steps, the final stream + moments), keeping the
+ * intermediate state in LDS, where the lattice allows it: D3Q19 with
+ * LBM_F32 or LBM_F32_DEV, LBM_BB_NONE or LBM_BB_REFERENCE, with or without
+ * force and tau fields and pressure faces; D2Q9 periodic with uniform force
+ * and tau, its fused steps only (under PAIR_FORCE also LBM_F16_DEV and
+ * LBM_BF16_DEV); Q * cells < 2^31. By default only where it measured faster
+ * (D3Q19: at least 2^21 cells, dims[0] and dims[1] at least 64, dims[2] one
+ * of 64, 96, ..., 256 or above 341; D2Q9: at least 2048^2 cells); PAIR_FORCE
+ * uses it wherever it applies, PAIR_OFF never. A single D3Q19 update
+ * (nsteps = 1; also 16-bit storage) is one launch unless PAIR_OFF. Results
+ * are bit-identical either way. */
 #define LBM_FLAG_PAIR_OFF 16
 #define LBM_FLAG_PAIR_FORCE 32
 

@@ -73,6 +73,53 @@ def _links(L, st, d, kern, m):
     return links
 
 
+def _two_step(kern: LBMKernel, fields: bool) -> bool:
+    """Whether lbm_step runs this lattice's sequences two units per launch
+    (the same choice, so the rollout's forward launches match its own)."""
+    return kern.D == 3 and kern.schedule(2, with_fields=fields)[1] > 0
+
+
+def _advance(L, st, d, kern, f0, wanted, nsteps, m, links, ff, tf, pairs, fN, moments):
+    """The forward pass of an nsteps rollout that materialises only the
+    post-collision states g_k with k in `wanted` (yielded as (k, g) in
+    order), then writes f_N and its moments. Units are the pull sequence's:
+    g_0 = C(f_0), g_k = C(S B g_{k-1}), f_N = S B g_{N-1}. With `pairs`,
+    runs of units between wanted states go two per launch (lbm_pull_phase 3
+    (SC, SC), 4 (C, SC), 5 (SC, S), 6 (C, S)), bit-identical to one per
+    launch."""
+    pool = [kern.empty(), kern.empty()]
+    cur, idx = f0, -1          # f_0 (idx -1) or g_idx
+    nslot = 0
+
+    def scratch():
+        nonlocal nslot
+        nslot += 1
+        return pool[nslot & 1]
+
+    stops = list(wanted)
+    last = nsteps - 2 if pairs else nsteps - 1   # the final launch starts from g_last
+    if last >= 0 and (not stops or stops[-1] != last):
+        stops.append(last)
+    for t in stops:
+        while idx < t:
+            two = pairs and t - idx >= 2
+            dst = kern.empty() if (idx + (2 if two else 1)) == t and t in wanted else scratch()
+            if two:
+                _pull(L, st, d, 4 if idx < 0 else 3, cur, dst, m, links, ff, tf)
+                idx += 2
+            else:
+                _pull(L, st, d, 0 if idx < 0 else 1, cur, dst, m, links, ff, tf)
+                idx += 1
+            cur = dst
+        if t in wanted:
+            yield t, cur
+    rho, u, p = moments
+    if pairs:
+        _pull(L, st, d, 6 if idx < 0 else 5, cur, fN, m, links, ff, tf, rho, u, p)
+    else:
+        _pull(L, st, d, 2, cur, fN, m, links, ff, tf, rho, u, p)
+
+
 class _Rollout(torch.autograd.Function):
     """The forward pass runs the pull sequence of lbm_step launch by launch:
     g_0 = C(f_0), g_k = C(S B g_{k-1}), f_N = S B g_{N-1}. The starting state
@@ -99,24 +146,20 @@ class _Rollout(torch.autograd.Function):
         ckpts = [f0]           # source of each segment's first state
         # the last segment's states come straight from this pass: no recompute
         s_last = ((nsteps - 1) // seg) * seg
-        tail = []
-        pool = [kern.empty(), kern.empty()]
-        prev = f0
-        for k in range(nsteps):
-            ckpt = (k + 1) % seg == 0 and k + 1 < nsteps   # g_k starts segment k + 1
-            in_tail = s_last <= k <= nsteps - 2
-            g = kern.empty() if (ckpt or in_tail) else pool[k & 1]
-            _pull(L, st, d, 0 if k == 0 else 1, prev, g, m, links, ff, tf)
-            if ckpt:
-                ckpts.append(g)
-            if in_tail:
-                tail.append(g)
-            prev = g
+        keep = {k for k in range(nsteps - 1) if (k + 1) % seg == 0}   # g_k starts segment k + 1
+        tail_idx = set(range(s_last, nsteps - 1))
         fN = kern.empty()
         rho = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
         u = torch.empty(kern.grid_shape + (kern.D,), dtype=torch.float32, device=dev)
         p = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
-        _pull(L, st, d, 2, prev, fN, m, links, ff, tf, rho, u, p)
+        pairs = _two_step(kern, ff is not None or tf is not None)
+        tail = []
+        for k, g in _advance(L, st, d, kern, f0, sorted(keep | tail_idx), nsteps, m, links, ff,
+                             tf, pairs, fN, (rho, u, p)):
+            if k in keep:
+                ckpts.append(g)
+            if k in tail_idx:
+                tail.append(g)
         ctx.kern = kern
         ctx.links = links
         ctx.nsteps = nsteps

@@ -15,3 +15,65 @@ def test_rollout_auto_checkpoint_choice():
     assert resolve_checkpoint(state, edge + 1, "auto", free_bytes=free) == "sqrt"
     for other in ("sqrt", "all", 7):
         assert resolve_checkpoint(state, 500, other, free_bytes=0) == other
+
+
+def _simulate(nsteps, wanted, pairs, monkeypatch):
+    """Run adjoint._advance with a recording stand-in for lbm_pull_phase and
+    replay its launches symbolically: each state is ('f', 0) for f_0, ('g', k)
+    for post-collision g_k, ('F', N) for f_N."""
+    from maddening_amd import adjoint
+
+    class Buf:
+        n = 0
+
+        def __init__(self):
+            Buf.n += 1
+            self.id = Buf.n
+            self.val = None
+
+    class Kern:
+        def empty(self):
+            return Buf()
+
+    log = []
+    monkeypatch.setattr(adjoint, "_pull", lambda L, st, d, ph, src, dst, *a: log.append((ph, src, dst)))
+    f0, fN = Buf(), Buf()
+    f0.val = ("f", -1)
+    got = {}
+    for k, g in adjoint._advance(None, None, None, Kern(), f0, wanted, nsteps, None, None,
+                                 None, None, pairs, fN, (None, None, None)):
+        got[k] = g
+    units = {0: 1, 1: 1, 2: 1, 3: 2, 4: 2, 5: 2, 6: 2}
+    for ph, src, dst in log:
+        assert src is not dst
+        assert src.val is not None, "a launch read a state nothing wrote"
+        kind, k = src.val
+        assert (kind == "f") == (ph in (0, 4, 6)), (ph, src.val)
+        k2 = k + units[ph]
+        dst.val = ("F", k2) if ph in (2, 5, 6) else ("g", k2)
+    for k, g in got.items():
+        assert g.val == ("g", k)          # still holds g_k: not overwritten later
+    assert fN.val == ("F", nsteps)
+    assert sorted(got) == sorted(wanted)
+    return [ph for ph, _, _ in log]
+
+
+def test_rollout_forward_launch_plan(monkeypatch):
+    """The rollout's forward pass keeps exactly the states it is asked for and
+    ends with f_N; with two-step launches every run between kept states goes
+    two units per launch (phases 3-6), with at most one single launch per run."""
+    import math
+    for nsteps in (1, 2, 3, 4, 5, 9, 10, 23, 500):
+        seg = max(1, math.ceil(math.sqrt(nsteps)))
+        s_last = ((nsteps - 1) // seg) * seg
+        wanted = sorted({k for k in range(nsteps - 1) if (k + 1) % seg == 0}
+                        | set(range(s_last, nsteps - 1)))
+        single = _simulate(nsteps, wanted, False, monkeypatch)
+        assert len(single) == nsteps + 1 and set(single) <= {0, 1, 2}
+        two = _simulate(nsteps, wanted, True, monkeypatch)
+        runs = len([k for k in wanted]) + 1
+        assert sum(1 for ph in two if ph in (0, 1)) <= runs
+        assert len(two) < len(single) or nsteps <= 2
+        # all states wanted: nothing to fold but the ends
+        every = list(range(nsteps - 1))
+        _simulate(nsteps, every, True, monkeypatch)

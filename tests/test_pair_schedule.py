@@ -58,20 +58,18 @@ def test_auto_keeps_single_launches_where_pairs_measured_slower():
     # 16-bit and 64-bit storage; fewer than 2^21 cells (96^3 and 100^3
     # measured slower with two-step launches); rows not a multiple of 32
     # (200x200x100 broke even), shorter than 64 or longer than 256 cells;
-    # a or b thinner than 64 cells; walls, pressure faces, force or tau
-    # fields, the push scheme, PAIR_OFF
+    # a or b thinner than 64 cells; halfway bounce-back, the push scheme,
+    # PAIR_OFF
     cases = [_desc((256, 256, 256), storage=2), _desc((256, 256, 256), storage=3),
              _desc((256, 256, 256), storage=1), _desc((96, 96, 96)), _desc((100, 100, 100)),
              _desc((127, 128, 128)), _desc((16, 16, 256)), _desc((200, 200, 100)),
              _desc((256, 256, 100)), _desc((1024, 64, 32)), _desc((256, 128, 288)),
              _desc((256, 256, 320)), _desc((32768, 1, 64)), _desc((2, 4096, 256)),
              _desc((64, 64, 500)), _desc((32, 128, 1024)), _desc((128, 16, 1024)),
-             _desc((63, 256, 256)), _desc((256, 63, 256)), _desc((256, 256, 256), bb=1),
-             _desc((256, 256, 256), faces=True),
+             _desc((63, 256, 256)), _desc((256, 63, 256)), _desc((256, 256, 256), bb=2),
              _desc((256, 256, 256), flags=_lib.LBM_FLAG_PAIR_OFF)]
     for d in cases:
         assert _schedule(d, 100) == (101, 0)
-    assert _schedule(_desc((256, 256, 256)), 100, with_fields=1) == (101, 0)
     d = _desc((256, 256, 256))
     d.scheme = _lib.LBM_SCHEME_PUSH
     assert _schedule(d, 100) == (101, 0)
@@ -168,3 +166,13 @@ def test_single_update_is_one_launch():
     d = _desc((64, 32, 32), flags=_lib.LBM_FLAG_PAIR_OFF)
     d.scheme = _lib.LBM_SCHEME_PULL
     assert _schedule(d, 1) == (2, 0)
+
+
+@pytest.mark.parametrize("dims", [(128, 128, 256), (256, 256, 256)])
+def test_auto_general_pairs_with_walls_fields_faces(dims):
+    """The reference wall rule, force or tau fields and pressure faces take
+    the march's general form under the same rule (the 128x128x256 sphere
+    channel: 0.090 against 0.120 ms per step)."""
+    for d, fields in [(_desc(dims, bb=1), 0), (_desc(dims, faces=True), 0), (_desc(dims), 1),
+                      (_desc(dims, bb=1, faces=True), 1)]:
+        assert _schedule(d, 500, with_fields=fields) == _folded(500)
