@@ -131,7 +131,9 @@ def tune_kernel(kern: LBMKernel, nsteps: int = 8, reps: int = 3, use_cache: bool
 def tune_batch(batch, nsteps: int = 10, reps: int = 3, use_cache: bool = True,
                grids: Optional[list] = None) -> dict:
     """Pick cells_per_thread (tile = 256 * cpt cells) and the persistent grid
-    size for a LatticeBatch; re-plans the batch with the winner."""
+    size for a LatticeBatch; re-plans the batch with the winner and sets
+    ``batch.grid``, which ``run`` and ``graph`` then use by default (on a
+    cache hit too)."""
     dev = torch.cuda.get_device_name(batch.device)
     sig = [(k.lat.name, k.grid_shape, k.bb) for k in batch.kernels]
     # a digest that is the same in every process (str hashes are salted)
@@ -141,6 +143,7 @@ def tune_batch(batch, nsteps: int = 10, reps: int = 3, use_cache: bool = True,
         hit = _load().get(key)
         if hit:
             batch.plan(hit["cells_per_thread"])
+            batch.grid = int(hit["grid"])
             return hit
     cus = torch.cuda.get_device_properties(batch.device).multi_processor_count
     results = []
@@ -153,6 +156,7 @@ def tune_batch(batch, nsteps: int = 10, reps: int = 3, use_cache: bool = True,
             results.append({"cells_per_thread": cpt, "grid": int(g), "ms_per_launch": ms})
     best = min(results, key=lambda r: r["ms_per_launch"])
     batch.plan(best["cells_per_thread"])
+    batch.grid = int(best["grid"])   # run() and graph() default to it
     rec = dict(best, candidates=results)
     _store(key, rec)
     return rec

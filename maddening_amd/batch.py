@@ -118,8 +118,14 @@ class LatticeBatch:
         self._handle = handle
         self.cells_per_thread = int(cells_per_thread)
         self.total_tiles = int(tiles.value)
+        # a grid tuned for the previous tiling does not carry over
+        self.grid = None
 
     def default_grid(self) -> int:
+        """The grid `run` and `graph` use when none is given: the one
+        autotune.tune_batch picked for this tiling, else 8 workgroups per CU."""
+        if getattr(self, "grid", None):
+            return int(min(self.total_tiles, self.grid))
         cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         return int(min(self.total_tiles, 8 * cus))
 

@@ -77,3 +77,45 @@ def test_rollout_forward_launch_plan(monkeypatch):
         # all states wanted: nothing to fold but the ends
         every = list(range(nsteps - 1))
         _simulate(nsteps, every, True, monkeypatch)
+
+
+def test_tune_batch_cache_hit_restores_tiling_and_grid(monkeypatch):
+    """A cached batch tuning re-plans the tiles and sets the grid that
+    LatticeBatch.run and .graph default to (CPU: a stand-in batch)."""
+    import torch
+    from maddening_amd import autotune
+    from maddening_amd.batch import LatticeBatch
+
+    class FakeBatch:
+        kernels, cells, device = [], 123, "cuda:0"
+        total_tiles = 10_000
+        default_grid = LatticeBatch.default_grid
+
+        def __init__(self):
+            self.planned = []
+            self.grid = None
+
+        def plan(self, cpt):
+            self.planned.append(cpt)
+            self.grid = None
+
+    monkeypatch.setattr(torch.cuda, "get_device_name", lambda dev=None: "fake")
+    monkeypatch.setattr(autotune, "_load",
+                        lambda: _AnyKey({"cells_per_thread": 2, "grid": 4096, "ms_per_launch": 1.0}))
+    b = FakeBatch()
+    rec = autotune.tune_batch(b)
+    assert rec["grid"] == 4096 and b.planned == [2] and b.grid == 4096
+    assert b.default_grid() == 4096
+    b.total_tiles = 1000
+    assert b.default_grid() == 1000       # never more workgroups than tiles
+
+
+class _AnyKey(dict):
+    """A cache stand-in that answers every key with one record."""
+
+    def __init__(self, rec):
+        super().__init__()
+        self.rec = rec
+
+    def get(self, key, default=None):
+        return self.rec
