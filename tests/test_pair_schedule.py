@@ -43,12 +43,12 @@ def test_auto_pairs_at_256_cubed(storage, nsteps):
 # lattices where two-step launches measured faster than the best one-step
 # launch shape: at least 2^21 cells, a and b at least 64 cells, whole rows
 # of 64 to 256 cells in multiples of 32, or rows of more than 341 cells in
-# 64-column tiles
+# 64-column tiles that cover the row with at most 10 % of extra columns
 @pytest.mark.parametrize("dims", [(128, 128, 128), (64, 256, 128), (256, 128, 64),
                                   (160, 160, 160), (192, 192, 192), (256, 256, 160),
                                   (320, 320, 192), (512, 256, 128), (64, 512, 64),
                                   (128, 128, 512), (256, 256, 512), (64, 64, 2048),
-                                  (96, 96, 1000), (64, 96, 342)])
+                                  (96, 96, 1000), (64, 96, 352), (64, 96, 700)])
 def test_auto_pairs_on_measured_winning_shapes(dims):
     assert _schedule(_desc(dims), 9) == (5, 5)
 
@@ -67,6 +67,7 @@ def test_auto_keeps_single_launches_where_pairs_measured_slower():
              _desc((256, 256, 320)), _desc((32768, 1, 64)), _desc((2, 4096, 256)),
              _desc((64, 64, 500)), _desc((32, 128, 1024)), _desc((128, 16, 1024)),
              _desc((63, 256, 256)), _desc((256, 63, 256)), _desc((256, 256, 256), bb=2),
+             _desc((64, 96, 342)), _desc((64, 96, 400)),   # tiles 12 % / 12 % over the row
              _desc((256, 256, 256), flags=_lib.LBM_FLAG_PAIR_OFF)]
     for d in cases:
         assert _schedule(d, 100) == (101, 0)
