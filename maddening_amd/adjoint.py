@@ -120,6 +120,159 @@ def _advance(L, st, d, kern, f0, wanted, nsteps, m, links, ff, tf, pairs, fN, mo
         _pull(L, st, d, 2, cur, fN, m, links, ff, tf, rho, u, p)
 
 
+def _forward_impl(ctx, f0, force, tau_field, rho_in, rho_out, kern: LBMKernel, nsteps: int,
+                  mask, seg: int, face_names):
+    """The rollout's forward pass: runs the pull sequence, keeps the states
+    the backward needs (checkpoints and the last segment), and records what
+    the backward needs on `ctx` (an autograd context or any object).
+    Returns ((f_N, rho, u, p), tensors to keep for the backward)."""
+    L = _lib.lib()
+    dev = kern.device
+    st = _stream(dev)
+    # a uniform force goes to the kernels as host numbers: read it once
+    fk = force.tolist() if (force is not None and force.dim() == 1) else force
+    ff, fu = kern._field(fk)
+    faces = _faces(face_names, rho_in, rho_out)
+    d = kern.desc(force=fu, faces=faces)
+    m = kern._mask(mask)
+    if kern.bb != "none" and m is None:
+        raise ValueError(f"bb='{kern.bb}' needs a wall mask")
+    tf = None if tau_field is None else tau_field.to(dev, torch.float32).contiguous()
+    links = _links(L, st, d, kern, m)
+    ckpts = [f0]           # source of each segment's first state
+    # the last segment's states come straight from this pass: no recompute
+    s_last = ((nsteps - 1) // seg) * seg
+    keep = {k for k in range(nsteps - 1) if (k + 1) % seg == 0}   # g_k starts segment k + 1
+    tail_idx = set(range(s_last, nsteps - 1))
+    fN = kern.empty()
+    rho = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
+    u = torch.empty(kern.grid_shape + (kern.D,), dtype=torch.float32, device=dev)
+    p = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
+    pairs = _two_step(kern, ff is not None or tf is not None)
+    tail = []
+    for k, g in _advance(L, st, d, kern, f0, sorted(keep | tail_idx), nsteps, m, links, ff,
+                         tf, pairs, fN, (rho, u, p)):
+        if k in keep:
+            ckpts.append(g)
+        if k in tail_idx:
+            tail.append(g)
+    ctx.kern = kern
+    ctx.links = links
+    ctx.nsteps = nsteps
+    ctx.seg = seg
+    ctx.mask = mask
+    ctx.force_is_field = force is not None and force.dim() > 1
+    ctx.force_dtype = None if force is None else force.dtype
+    ctx.force_list = fk if isinstance(fk, list) else None
+    ctx.n_ckpt = len(ckpts)
+    ctx.faces = faces
+    ctx.rho_dtypes = (None if rho_in is None else rho_in.dtype,
+                      None if rho_out is None else rho_out.dtype)
+    return (fN, rho, u, p), [fN, force, tau_field, *ckpts, *tail]
+
+
+def _backward_impl(ctx, saved, needs, g_f, g_rho, g_u, g_p):
+    """The rollout's backward pass from what _forward_impl recorded on `ctx`
+    and kept in `saved`; `needs` flags the gradients wanted, for (f_0,
+    force, tau_field, rho_in, rho_out). Returns those five gradients (None
+    where not wanted) followed by five Nones."""
+    kern: LBMKernel = ctx.kern
+    f_N, force, tau_field, *rest = saved
+    ckpts, tail = rest[:ctx.n_ckpt], rest[ctx.n_ckpt:]
+    L = _lib.lib()
+    dev = kern.device
+    st = _stream(dev)
+    if ctx.force_list is not None:
+        ff, fu = None, ctx.force_list
+    else:
+        ff, fu = kern._field(force)
+    d = kern.desc(force=fu, faces=ctx.faces)
+    m = kern._mask(ctx.mask)
+    tf = None if tau_field is None else tau_field.to(dev, torch.float32).contiguous()
+    need_f0, need_force, need_tau = needs[0], needs[1], needs[2]
+    work = torch.empty(L.lbm_adjoint_workspace_size(ctypes.byref(d)), dtype=torch.uint8,
+                       device=dev)
+    _lib.check(L.lbm_adjoint_begin(st, ctypes.byref(d), _ptr(work)), "lbm_adjoint_begin")
+    tau_bar = torch.zeros(kern.grid_shape, dtype=torch.float32, device=dev) \
+        if (need_tau and tf is not None) else None
+    fbar_field = torch.zeros(kern.grid_shape + (kern.D,), dtype=torch.float32, device=dev) \
+        if (need_force and ctx.force_is_field) else None
+
+    def c(t):
+        return None if t is None else t.to(dev, torch.float32).contiguous()
+
+    # Under the reference wall rule the cotangents between steps are kept
+    # with the slots of wall cells exchanged, so each transposed step is a
+    # plain gather (no neighbour mask reads); the last step writes dL/df_0
+    # in the ordinary form.
+    # With Zou-He faces every cotangent but dL/df_0 is passed back through
+    # the faces' transpose (their states came out of the faces), which
+    # also accumulates dL/drho of each face.
+    faces_on = ctx.faces is not None
+    OUTF = _lib.LBM_FLAG_ADJ_OUT_FACES if faces_on else 0
+
+    def with_flags(bits, extra=0):
+        dd = _lib.LbmDesc.from_buffer_copy(d)
+        if kern.bb == "reference":
+            dd.flags |= bits
+        dd.flags |= extra
+        return dd
+    d_mom = with_flags(_lib.LBM_FLAG_ADJ_OUT_SWAPPED, OUTF)
+    d_mid = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED | _lib.LBM_FLAG_ADJ_OUT_SWAPPED, OUTF)
+    d_first = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED)
+    fbar = torch.empty(kern.state_shape, dtype=kern.dtype, device=dev)
+    gin = None if g_f is None else g_f.to(dev, kern.dtype).contiguous()
+    _lib.check(L.lbm_moments_adjoint(st, ctypes.byref(d_mom), _ptr(f_N), _ptr(m), _ptr(ff),
+                                     _ptr(c(g_rho)), _ptr(c(g_u)), _ptr(c(g_p)), _ptr(gin),
+                                     _ptr(fbar), _ptr(fbar_field), _ptr(work)),
+               "lbm_moments_adjoint")
+    # Per segment (last first): recompute g_s .. g_{s+n-2} from the
+    # checkpoint with the forward's pull launches, then transpose steps
+    # s+n-1 .. s; step k's starting state is f_0 or gathered from g_{k-1}.
+    spare = torch.empty_like(fbar)
+    for si in reversed(range(len(ckpts))):
+        s = si * ctx.seg
+        n = min(ctx.seg, ctx.nsteps - s)
+        states = [ckpts[si]]
+        if si == len(ckpts) - 1:
+            states += list(tail)
+        for i in range(len(states), n):
+            g = kern.empty()
+            _pull(L, st, d, 1 if (s > 0 or i > 1) else 0, states[-1], g, m, ctx.links, ff, tf)
+            states.append(g)
+        for i in reversed(range(n)):
+            post = int(s > 0 or i > 0)
+            dk = d_mid if post else d_first
+            _lib.check(L.lbm_adjoint_step(st, ctypes.byref(dk), _ptr(states[i]), post,
+                                          _ptr(fbar), _ptr(spare), _ptr(m),
+                                          _ptr(ctx.links), _ptr(ff), _ptr(tf),
+                                          _ptr(tau_bar), _ptr(fbar_field), _ptr(work)),
+                       "lbm_adjoint_step")
+            fbar, spare = spare, fbar
+        del states
+    grad_force = None
+    if need_force and force is not None:
+        if ctx.force_is_field:
+            grad_force = fbar_field.to(ctx.force_dtype)
+        else:
+            out = torch.empty(3, dtype=torch.float64, device=dev)
+            _lib.check(L.lbm_adjoint_end(st, ctypes.byref(d), _ptr(work), _ptr(out)),
+                       "lbm_adjoint_end")
+            grad_force = out[: kern.D].to(ctx.force_dtype)
+    grad_tau = tau_bar.to(tau_field.dtype) if tau_bar is not None else None
+    grad_rin = grad_rout = None
+    if faces_on and (needs[3] or needs[4]):
+        rb = torch.empty(2, dtype=torch.float64, device=dev)
+        _lib.check(L.lbm_adjoint_face_end(st, ctypes.byref(d), _ptr(work), _ptr(rb)),
+                   "lbm_adjoint_face_end")
+        if needs[3]:
+            grad_rin = rb[0].to(ctx.rho_dtypes[0])
+        if needs[4]:
+            grad_rout = rb[1].to(ctx.rho_dtypes[1])
+    return ((fbar if need_f0 else None), grad_force, grad_tau, grad_rin, grad_rout,
+            None, None, None, None, None)
+
+
 class _Rollout(torch.autograd.Function):
     """The forward pass runs the pull sequence of lbm_step launch by launch:
     g_0 = C(f_0), g_k = C(S B g_{k-1}), f_N = S B g_{N-1}. The starting state
@@ -130,149 +283,14 @@ class _Rollout(torch.autograd.Function):
     def forward(ctx, f0, force, tau_field, rho_in, rho_out, kern: LBMKernel, nsteps: int, mask,
                 seg: int, face_names):
         ctx.set_materialize_grads(False)
-        L = _lib.lib()
-        dev = kern.device
-        st = _stream(dev)
-        # a uniform force goes to the kernels as host numbers: read it once
-        fk = force.tolist() if (force is not None and force.dim() == 1) else force
-        ff, fu = kern._field(fk)
-        faces = _faces(face_names, rho_in, rho_out)
-        d = kern.desc(force=fu, faces=faces)
-        m = kern._mask(mask)
-        if kern.bb != "none" and m is None:
-            raise ValueError(f"bb='{kern.bb}' needs a wall mask")
-        tf = None if tau_field is None else tau_field.to(dev, torch.float32).contiguous()
-        links = _links(L, st, d, kern, m)
-        ckpts = [f0]           # source of each segment's first state
-        # the last segment's states come straight from this pass: no recompute
-        s_last = ((nsteps - 1) // seg) * seg
-        keep = {k for k in range(nsteps - 1) if (k + 1) % seg == 0}   # g_k starts segment k + 1
-        tail_idx = set(range(s_last, nsteps - 1))
-        fN = kern.empty()
-        rho = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
-        u = torch.empty(kern.grid_shape + (kern.D,), dtype=torch.float32, device=dev)
-        p = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
-        pairs = _two_step(kern, ff is not None or tf is not None)
-        tail = []
-        for k, g in _advance(L, st, d, kern, f0, sorted(keep | tail_idx), nsteps, m, links, ff,
-                             tf, pairs, fN, (rho, u, p)):
-            if k in keep:
-                ckpts.append(g)
-            if k in tail_idx:
-                tail.append(g)
-        ctx.kern = kern
-        ctx.links = links
-        ctx.nsteps = nsteps
-        ctx.seg = seg
-        ctx.mask = mask
-        ctx.force_is_field = force is not None and force.dim() > 1
-        ctx.force_dtype = None if force is None else force.dtype
-        ctx.force_list = fk if isinstance(fk, list) else None
-        ctx.n_ckpt = len(ckpts)
-        ctx.faces = faces
-        ctx.rho_dtypes = (None if rho_in is None else rho_in.dtype,
-                          None if rho_out is None else rho_out.dtype)
-        ctx.save_for_backward(fN, force, tau_field, *ckpts, *tail)
-        return fN, rho, u, p
+        outs, keep = _forward_impl(ctx, f0, force, tau_field, rho_in, rho_out, kern, nsteps, mask,
+                                   seg, face_names)
+        ctx.save_for_backward(*keep)
+        return outs
 
     @staticmethod
     def backward(ctx, g_f, g_rho, g_u, g_p):
-        kern: LBMKernel = ctx.kern
-        f_N, force, tau_field, *rest = ctx.saved_tensors
-        ckpts, tail = rest[:ctx.n_ckpt], rest[ctx.n_ckpt:]
-        L = _lib.lib()
-        dev = kern.device
-        st = _stream(dev)
-        if ctx.force_list is not None:
-            ff, fu = None, ctx.force_list
-        else:
-            ff, fu = kern._field(force)
-        d = kern.desc(force=fu, faces=ctx.faces)
-        m = kern._mask(ctx.mask)
-        tf = None if tau_field is None else tau_field.to(dev, torch.float32).contiguous()
-        need_f0, need_force, need_tau = ctx.needs_input_grad[0], ctx.needs_input_grad[1], \
-            ctx.needs_input_grad[2]
-        work = torch.empty(L.lbm_adjoint_workspace_size(ctypes.byref(d)), dtype=torch.uint8,
-                           device=dev)
-        _lib.check(L.lbm_adjoint_begin(st, ctypes.byref(d), _ptr(work)), "lbm_adjoint_begin")
-        tau_bar = torch.zeros(kern.grid_shape, dtype=torch.float32, device=dev) \
-            if (need_tau and tf is not None) else None
-        fbar_field = torch.zeros(kern.grid_shape + (kern.D,), dtype=torch.float32, device=dev) \
-            if (need_force and ctx.force_is_field) else None
-
-        def c(t):
-            return None if t is None else t.to(dev, torch.float32).contiguous()
-
-        # Under the reference wall rule the cotangents between steps are kept
-        # with the slots of wall cells exchanged, so each transposed step is a
-        # plain gather (no neighbour mask reads); the last step writes dL/df_0
-        # in the ordinary form.
-        # With Zou-He faces every cotangent but dL/df_0 is passed back through
-        # the faces' transpose (their states came out of the faces), which
-        # also accumulates dL/drho of each face.
-        faces_on = ctx.faces is not None
-        OUTF = _lib.LBM_FLAG_ADJ_OUT_FACES if faces_on else 0
-
-        def with_flags(bits, extra=0):
-            dd = _lib.LbmDesc.from_buffer_copy(d)
-            if kern.bb == "reference":
-                dd.flags |= bits
-            dd.flags |= extra
-            return dd
-        d_mom = with_flags(_lib.LBM_FLAG_ADJ_OUT_SWAPPED, OUTF)
-        d_mid = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED | _lib.LBM_FLAG_ADJ_OUT_SWAPPED, OUTF)
-        d_first = with_flags(_lib.LBM_FLAG_ADJ_IN_SWAPPED)
-        fbar = torch.empty(kern.state_shape, dtype=kern.dtype, device=dev)
-        gin = None if g_f is None else g_f.to(dev, kern.dtype).contiguous()
-        _lib.check(L.lbm_moments_adjoint(st, ctypes.byref(d_mom), _ptr(f_N), _ptr(m), _ptr(ff),
-                                         _ptr(c(g_rho)), _ptr(c(g_u)), _ptr(c(g_p)), _ptr(gin),
-                                         _ptr(fbar), _ptr(fbar_field), _ptr(work)),
-                   "lbm_moments_adjoint")
-        # Per segment (last first): recompute g_s .. g_{s+n-2} from the
-        # checkpoint with the forward's pull launches, then transpose steps
-        # s+n-1 .. s; step k's starting state is f_0 or gathered from g_{k-1}.
-        spare = torch.empty_like(fbar)
-        for si in reversed(range(len(ckpts))):
-            s = si * ctx.seg
-            n = min(ctx.seg, ctx.nsteps - s)
-            states = [ckpts[si]]
-            if si == len(ckpts) - 1:
-                states += list(tail)
-            for i in range(len(states), n):
-                g = kern.empty()
-                _pull(L, st, d, 1 if (s > 0 or i > 1) else 0, states[-1], g, m, ctx.links, ff, tf)
-                states.append(g)
-            for i in reversed(range(n)):
-                post = int(s > 0 or i > 0)
-                dk = d_mid if post else d_first
-                _lib.check(L.lbm_adjoint_step(st, ctypes.byref(dk), _ptr(states[i]), post,
-                                              _ptr(fbar), _ptr(spare), _ptr(m),
-                                              _ptr(ctx.links), _ptr(ff), _ptr(tf),
-                                              _ptr(tau_bar), _ptr(fbar_field), _ptr(work)),
-                           "lbm_adjoint_step")
-                fbar, spare = spare, fbar
-            del states
-        grad_force = None
-        if need_force and force is not None:
-            if ctx.force_is_field:
-                grad_force = fbar_field.to(ctx.force_dtype)
-            else:
-                out = torch.empty(3, dtype=torch.float64, device=dev)
-                _lib.check(L.lbm_adjoint_end(st, ctypes.byref(d), _ptr(work), _ptr(out)),
-                           "lbm_adjoint_end")
-                grad_force = out[: kern.D].to(ctx.force_dtype)
-        grad_tau = tau_bar.to(tau_field.dtype) if tau_bar is not None else None
-        grad_rin = grad_rout = None
-        if faces_on and (ctx.needs_input_grad[3] or ctx.needs_input_grad[4]):
-            rb = torch.empty(2, dtype=torch.float64, device=dev)
-            _lib.check(L.lbm_adjoint_face_end(st, ctypes.byref(d), _ptr(work), _ptr(rb)),
-                       "lbm_adjoint_face_end")
-            if ctx.needs_input_grad[3]:
-                grad_rin = rb[0].to(ctx.rho_dtypes[0])
-            if ctx.needs_input_grad[4]:
-                grad_rout = rb[1].to(ctx.rho_dtypes[1])
-        return ((fbar if need_f0 else None), grad_force, grad_tau, grad_rin, grad_rout,
-                None, None, None, None, None)
+        return _backward_impl(ctx, ctx.saved_tensors, ctx.needs_input_grad, g_f, g_rho, g_u, g_p)
 
 
 def _faces(names, rho_in, rho_out):

@@ -8,8 +8,12 @@ This is the PyTorch counterpart of the jax.ffi target the reference's
   tracing see the op without running it;
 * an autograd formula. It returns the gradients of f_0, of the force
   (uniform (D,) or per cell (*G, D)) and of the per-cell tau field through
-  the HIP adjoint (``maddening_amd.adjoint.rollout``, sqrt(N)
-  checkpointing), recomputing the forward from the saved inputs.
+  the HIP adjoint (``maddening_amd.adjoint``, sqrt(N) checkpointing). When
+  an input requires grad, the op's forward is the rollout's own forward
+  pass (the same launches and bits as ``lbm_step``), and the states it keeps
+  go to the backward with the output, so the backward does not re-run the
+  forward; where they cannot be handed over (a traced graph, say) the
+  backward recomputes them from the saved inputs.
 
 Arguments: ``f`` is the SoA state (Q, *G), float32 or float64. ``mask``
 is an optional (*G) uint8 wall mask. ``force`` is optional, (D,) or
@@ -47,12 +51,35 @@ def _force_arg(force: Optional[torch.Tensor]):
     return [float(v) for v in force.tolist()] if force.dim() == 1 else force
 
 
+def _rollout_force(force: Optional[torch.Tensor], device):
+    """The force as the rollout takes it: a uniform force in float64 (it goes
+    to the kernels as host numbers either way), a field in its own dtype."""
+    if force is None:
+        return None
+    f = force.detach()
+    return f.to(device=device, dtype=torch.float64 if f.dim() == 1 else f.dtype)
+
+
 @torch.library.custom_op("maddening_amd::lbm_step", mutates_args=())
 def lbm_step(f: torch.Tensor, mask: Optional[torch.Tensor], force: Optional[torch.Tensor],
              tau_field: Optional[torch.Tensor], tau: float, bb: str, nsteps: int,
              lid_velocity: list[float]) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor,
                                                  torch.Tensor]:
     k = _kernel(f, tau, bb, lid_velocity)
+    if any(t is not None and t.requires_grad for t in (f, force, tau_field)):
+        # a gradient may follow: run the rollout's forward, which keeps its
+        # checkpoints, and pass them to the backward on the first output
+        from types import SimpleNamespace
+
+        from .adjoint import _forward_impl, segment_length
+        rec = SimpleNamespace()
+        outs, keep = _forward_impl(rec, f.detach().contiguous(), _rollout_force(force, k.device),
+                                   None if tau_field is None else tau_field.detach(), None,
+                                   None, k, int(nsteps), mask, segment_length(int(nsteps), "sqrt"),
+                                   None)
+        rec.saved = keep
+        outs[0]._lbm_rollout = rec
+        return outs
     fN, rho, u, p = k.step(f.contiguous(), int(nsteps), mask=mask, force=_force_arg(force),
                            tau_field=tau_field)
     return fN, rho, u, p
@@ -70,14 +97,27 @@ def _setup_context(ctx, inputs, output):
     f, mask, force, tau_field, tau, bb, nsteps, lid = inputs
     ctx.save_for_backward(f, mask, force, tau_field)
     ctx.args = (float(tau), bb, int(nsteps), list(lid))
+    # the forward pass's kept states, when the op's forward ran the rollout
+    ctx.rollout = output[0].__dict__.pop("_lbm_rollout", None)
 
 
 def _backward(ctx, g_f, g_rho, g_u, g_p):
-    from .adjoint import rollout
+    from .adjoint import _backward_impl, rollout
     f, mask, force, tau_field = ctx.saved_tensors
     tau, bb, nsteps, lid = ctx.args
     k = _kernel(f, tau, bb, lid)
     need_f, _, need_force, need_tau = ctx.needs_input_grad[:4]
+    rec = ctx.rollout
+    if rec is not None:
+        ctx.rollout = None
+        gf, gF, gT = _backward_impl(rec, rec.saved, (need_f, need_force and force is not None,
+                                                     need_tau and tau_field is not None,
+                                                     False, False), g_f, g_rho, g_u, g_p)[:3]
+        if gF is not None:
+            gF = gF.to(device=force.device, dtype=force.dtype)
+        if gT is not None:
+            gT = gT.to(tau_field.dtype)
+        return gf, None, gF, gT, None, None, None, None
     with torch.enable_grad():
         f0 = f.detach().requires_grad_(need_f)
         fr = None

@@ -71,3 +71,38 @@ def test_opcheck():
     args = (f, mask, torch.tensor([1e-5, 0.0, 0.0]), tau, 0.8, "reference", 3, [0.0, 0.0, 0.0])
     torch.library.opcheck(OP, args, test_utils=("test_schema", "test_autograd_registration",
                                                 "test_faketensor"))
+
+
+@pytest.mark.gpu
+def test_op_backward_does_not_rerun_the_forward(monkeypatch):
+    """With an input that requires grad the op's forward is the rollout's
+    forward; its kept states reach the backward, which then makes exactly
+    the launches of the rollout's own backward (segment recomputes and
+    transposed steps), not a second forward pass. The outputs still equal
+    LBMKernel.step bit for bit."""
+    from maddening_amd import adjoint
+    k, f, mask, tau = _inputs((8, 6, 10), torch.float32, seed=3)
+    b = k.step(f, 9, mask=mask, force=[1e-5, 0.0, 0.0], tau_field=tau)
+    calls = []
+    real = adjoint._pull
+    monkeypatch.setattr(adjoint, "_pull", lambda *a, **kw: (calls.append(a[3]), real(*a, **kw)))
+
+    def run(op):
+        t = tau.clone().requires_grad_(True)
+        if op:
+            outs = OP(f, mask, torch.tensor([1e-5, 0.0, 0.0]), t, 0.8, "reference", 9,
+                      [0.0, 0.0, 0.0])
+            assert not hasattr(outs[0], "_lbm_rollout")     # handed to the backward
+        else:
+            outs = adjoint.rollout(k, f, 9, force=torch.tensor([1e-5, 0.0, 0.0]), tau_field=t,
+                                   mask=mask)
+        n0 = len(calls)
+        outs[2].sum().backward()
+        torch.cuda.synchronize()
+        return outs, calls[n0:], t.grad
+    outs_op, back_op, g_op = run(True)
+    outs_ro, back_ro, g_ro = run(False)
+    for x, y in zip(outs_op, b):
+        assert torch.equal(x, y)
+    assert back_op == back_ro
+    torch.testing.assert_close(g_op, g_ro, rtol=0, atol=0)
