@@ -308,9 +308,9 @@ int lbm_batch_plan_create(void* stream, const lbm_batch_item* items, int nitems,
 
 /*
  * Two steps per launch for the periodic D3Q19 lattices of a plan (rows of
- * at most 341 cells): on, or off (the default after lbm_batch_plan_create:
- * on the 64-lattice benchmark batch, whose 32 periodic cubes have edges of
- * 32-160 cells, it measured 41.1k MLUPS against 41.2k).
+ * at most 341 cells): on (the default after lbm_batch_plan_create: the
+ * 64-lattice benchmark batch, whose 32 periodic cubes have edges of 32-160
+ * cells, runs at 44.8k MLUPS against 40.5k with one step per launch), or off.
  * Their fused steps then run two per launch, in an even number of launches,
  * so every lattice ends in the same buffer with the same bits as with one
  * launch per step. Takes effect for later lbm_batch_run and

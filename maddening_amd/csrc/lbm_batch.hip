@@ -203,7 +203,9 @@ struct lbm_batch_plan_t {
   const void* dplan;
   std::vector<mlbm::BatchGroup> groups;
   std::vector<mlbm::LinkJob> links;
-  int pairs = 0;  // run two-step classes two steps per launch (lbm_batch_plan_pairs)
+  // run two-step classes two steps per launch (lbm_batch_plan_pairs); on by
+  // default: the 64-lattice benchmark batch ran at 44.8k MLUPS against 40.5k
+  int pairs = 1;
 };
 
 namespace mlbm {
