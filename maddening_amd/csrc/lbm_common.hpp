@@ -52,15 +52,25 @@ __device__ __forceinline__ constexpr C weight(int q) {
 template <class T> struct Storage;
 
 // Each trait: load (stored value -> deviation), enc (deviation -> stored
-// value), put (non-temporal store of a stored value) and store = put(enc).
+// value), quant (deviation -> the deviation a store and reload gives back,
+// for the storage types that hold deviations), put (non-temporal store of a
+// stored value) and store = put(enc).
 template <> struct Storage<float> {
   using C = float;
+  // quant(d) equals a store and reload of d only where Sterbenz holds
+  static constexpr bool kQuantExact = false;
   template <class L>
   static __device__ __forceinline__ float load(const float* p, int q) {
     return *p - weight<L, float>(q);
   }
   template <class L>
   static __device__ __forceinline__ float enc(int q, float d) { return weight<L, float>(q) + d; }
+  // the deviation a store and reload gives back: (w + d) - w, exact in
+  // float32 for |d| < w
+  template <class L>
+  static __device__ __forceinline__ float quant(int q, float d) {
+    return (weight<L, float>(q) + d) - weight<L, float>(q);
+  }
   static __device__ __forceinline__ void put(float* p, float v) { __builtin_nontemporal_store(v, p); }
   template <class L>
   static __device__ __forceinline__ void store(float* p, int q, float d) { put(p, enc<L>(q, d)); }
@@ -68,12 +78,15 @@ template <> struct Storage<float> {
 
 template <> struct Storage<double> {
   using C = double;
+  static constexpr bool kQuantExact = false;  // no conservation step (float64)
   template <class L>
   static __device__ __forceinline__ double load(const double* p, int q) {
     return *p - weight<L, double>(q);
   }
   template <class L>
   static __device__ __forceinline__ double enc(int q, double d) { return weight<L, double>(q) + d; }
+  template <class L>
+  static __device__ __forceinline__ double quant(int, double d) { return d; }
   static __device__ __forceinline__ void put(double* p, double v) { __builtin_nontemporal_store(v, p); }
   template <class L>
   static __device__ __forceinline__ void store(double* p, int q, double d) { put(p, enc<L>(q, d)); }
@@ -87,12 +100,15 @@ struct f32dev_t {
 
 template <> struct Storage<f32dev_t> {
   using C = float;
+  static constexpr bool kQuantExact = true;
   template <class L>
   static __device__ __forceinline__ float load(const f32dev_t* p, int) {
     return p->v;
   }
   template <class L>
   static __device__ __forceinline__ f32dev_t enc(int, float d) { return f32dev_t{d}; }
+  template <class L>
+  static __device__ __forceinline__ float quant(int, float d) { return d; }
   static __device__ __forceinline__ void put(f32dev_t* p, f32dev_t v) {
     __builtin_nontemporal_store(v.v, &p->v);
   }
@@ -107,12 +123,18 @@ constexpr float kHalfScale = 1024.0f;
 
 template <> struct Storage<_Float16> {
   using C = float;
+  static constexpr bool kQuantExact = true;
   template <class L>
   static __device__ __forceinline__ float load(const _Float16* p, int) {
     return (float)(*p) * (1.0f / kHalfScale);
   }
   template <class L>
   static __device__ __forceinline__ _Float16 enc(int, float d) { return (_Float16)(d * kHalfScale); }
+  // the deviation a store and reload of d gives back
+  template <class L>
+  static __device__ __forceinline__ float quant(int, float d) {
+    return (float)(_Float16)(d * kHalfScale) * (1.0f / kHalfScale);
+  }
   static __device__ __forceinline__ void put(_Float16* p, _Float16 v) {
     __builtin_nontemporal_store(v, p);
   }
@@ -122,12 +144,15 @@ template <> struct Storage<_Float16> {
 
 template <> struct Storage<__bf16> {
   using C = float;
+  static constexpr bool kQuantExact = true;
   template <class L>
   static __device__ __forceinline__ float load(const __bf16* p, int) {
     return (float)(*p);
   }
   template <class L>
   static __device__ __forceinline__ __bf16 enc(int, float d) { return (__bf16)d; }
+  template <class L>
+  static __device__ __forceinline__ float quant(int, float d) { return (float)(__bf16)d; }
   static __device__ __forceinline__ void put(__bf16* p, __bf16 v) { __builtin_nontemporal_store(v, p); }
   template <class L>
   static __device__ __forceinline__ void store(__bf16* p, int q, float d) { put(p, enc<L>(q, d)); }
@@ -176,30 +201,64 @@ __device__ __forceinline__ float recip<float>(float x) {
 }
 
 // ---------------------------------------------------------------------------
+// Sums over the populations, each accumulated in q order from its first
+// term: sum d, and sum d e along each internal axis.
+// ---------------------------------------------------------------------------
+template <class L, class C>
+__device__ __forceinline__ C mass_sum(const C (&d)[L::Q]) {
+  C s = d[0];
+#pragma unroll
+  for (int q = 1; q < L::Q; ++q) s += d[q];
+  return s;
+}
+
+template <class L>
+__host__ __device__ constexpr int ecomp(int axis, int q) {
+  return axis == 0 ? L::ea[q] : (axis == 1 ? L::eb[q] : L::ec[q]);
+}
+
+// sum over q != skip of e_q,axis d_q (0 when no term is left)
+template <class L, int AXIS, int SKIP, class C>
+__device__ __forceinline__ C axis_sum(const C (&d)[L::Q]) {
+  C s = C(0);
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < L::Q; ++q) {
+    if (q == SKIP) continue;
+    eterm(ecomp<L>(AXIS, q), d[q], s, any);
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
 // Macroscopic moments with the Guo half-force correction
 // (lbm.py:209-239): rho = sum f, u = (sum f e + F/2) / max(rho, 1e-10).
 // Computed from deviations: rho = 1 + sum d (sum w = 1), sum f e = sum d e.
 // ---------------------------------------------------------------------------
-template <class L, class C>
+//
+// HASF = false: the caller's force is zero (uniform, no field). Every force
+// term is then an exact zero, so leaving them out changes no value (at most
+// the sign of a zero result), and a kernel that knows it has no force saves
+// their arithmetic and registers.
+template <class L, class C, bool HASF = true>
 __device__ __forceinline__ void moments(const C (&d)[L::Q], C Fa, C Fb, C Fc,
                                         C& drho, C& rho, C& ua, C& ub, C& uc) {
-  C s = 0, ja = 0, jb = 0, jc = 0;
-#pragma unroll
-  for (int q = 0; q < L::Q; ++q) {
-    s += d[q];
-    if (L::ea[q] > 0) ja += d[q];
-    if (L::ea[q] < 0) ja -= d[q];
-    if (L::eb[q] > 0) jb += d[q];
-    if (L::eb[q] < 0) jb -= d[q];
-    if (L::ec[q] > 0) jc += d[q];
-    if (L::ec[q] < 0) jc -= d[q];
-  }
+  const C s = mass_sum<L, C>(d);
+  const C ja = axis_sum<L, 0, -1, C>(d);
+  const C jb = axis_sum<L, 1, -1, C>(d);
+  const C jc = axis_sum<L, 2, -1, C>(d);
   drho = s;
   rho = C(1) + s;
   const C inv = recip(rho > C(1e-10) ? rho : C(1e-10));
-  ua = (ja + C(0.5) * Fa) * inv;
-  ub = (jb + C(0.5) * Fb) * inv;
-  uc = (jc + C(0.5) * Fc) * inv;
+  if constexpr (HASF) {
+    ua = (ja + C(0.5) * Fa) * inv;
+    ub = (jb + C(0.5) * Fb) * inv;
+    uc = (jc + C(0.5) * Fc) * inv;
+  } else {
+    ua = ja * inv;
+    ub = jb * inv;
+    uc = jc * inv;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -215,14 +274,6 @@ __device__ __forceinline__ void moments(const C (&d)[L::Q], C Fa, C Fb, C Fc,
 // - 1.5 u.u) and an odd part 3 rho e.u, and the source into the even
 // 9 (e.u)(e.F) - 3 u.F and the odd 3 e.F. Multiply-adds are written as
 // explicit fma so every kernel variant rounds identically.
-//
-// FIXUP (float32 storage): in exact arithmetic the collision keeps
-// sum d = rho - 1 and moves sum d e from j to j + F. Float32 rounding misses
-// both by about an ulp per step, which accumulates in rho and u over a run,
-// so the miss is projected back onto the conserved moments: adding
-// w_q (a + 3 e_q . b) changes sum d by a and sum d e by b, because sum w = 1,
-// sum w e = 0 and sum w e e = I/3. (With 16-bit storage the rounding of the
-// stored values dominates and the fix-up buys nothing.)
 // ---------------------------------------------------------------------------
 template <class C>
 __device__ __forceinline__ C fmadd(C a, C b, C c) {
@@ -233,32 +284,26 @@ __device__ __forceinline__ float fmadd<float>(float a, float b, float c) {
   return __builtin_fmaf(a, b, c);
 }
 
-template <class L, class C, bool FIXUP>
+template <class L, class C, bool HASF = true>
 __device__ __forceinline__ void collide_impl(C (&d)[L::Q], C inv_tau, C Fa, C Fb, C Fc, C drho,
                                              C rho, C ua, C ub, C uc) {
-  C ja = 0, jb = 0, jc = 0;
-  if constexpr (FIXUP) {
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) {
-      if (L::ea[q] > 0) ja += d[q];
-      if (L::ea[q] < 0) ja -= d[q];
-      if (L::eb[q] > 0) jb += d[q];
-      if (L::eb[q] < 0) jb -= d[q];
-      if (L::ec[q] > 0) jc += d[q];
-      if (L::ec[q] < 0) jc -= d[q];
-    }
-  }
   const C usq15 = C(1.5) * fmadd(ua, ua, fmadd(ub, ub, uc * uc));
-  const C uF3 = C(3) * fmadd(ua, Fa, fmadd(ub, Fb, uc * Fc));
   const C keep = C(1) - inv_tau;
-  const C src = C(1) - C(0.5) * inv_tau;
   const C rho3 = C(3) * rho;
   const C rho45 = C(4.5) * rho;
+  // without a force the source terms below are exact zeros: fma(a, b, +-0)
+  // rounds a * b once, as the product alone does
   // rest population: e = 0
   {
     const C w = weight<L, C>(0);
     const C A = fmadd(-rho, usq15, drho);
-    d[0] = fmadd(keep, d[0], fmadd(inv_tau * w, A, -(src * w) * uF3));
+    if constexpr (HASF) {
+      const C uF3 = C(3) * fmadd(ua, Fa, fmadd(ub, Fb, uc * Fc));
+      const C src = C(1) - C(0.5) * inv_tau;
+      d[0] = fmadd(keep, d[0], fmadd(inv_tau * w, A, -(src * w) * uF3));
+    } else {
+      d[0] = fmadd(keep, d[0], (inv_tau * w) * A);
+    }
   }
 #pragma unroll
   for (int q = 1; q < L::Q; ++q) {
@@ -266,48 +311,138 @@ __device__ __forceinline__ void collide_impl(C (&d)[L::Q], C inv_tau, C Fa, C Fb
     const int m = L::opp[q];
     const C w = weight<L, C>(q);
     const C eu = edot<L, C>(q, ua, ub, uc);
-    const C eF = edot<L, C>(q, Fa, Fb, Fc);
     const C A = fmadd(rho45 * eu, eu, fmadd(-rho, usq15, drho));  // even part of (feq - w)/w
     const C B = rho3 * eu;                                         // odd part
-    const C sw = src * w;
     const C wt = inv_tau * w;
-    const C even = fmadd(wt, A, sw * fmadd(C(9) * eu, eF, -uF3));
-    const C odd = fmadd(wt, B, (C(3) * sw) * eF);
+    C even, odd;
+    if constexpr (HASF) {
+      const C uF3 = C(3) * fmadd(ua, Fa, fmadd(ub, Fb, uc * Fc));
+      const C eF = edot<L, C>(q, Fa, Fb, Fc);
+      const C sw = (C(1) - C(0.5) * inv_tau) * w;
+      even = fmadd(wt, A, sw * fmadd(C(9) * eu, eF, -uF3));
+      odd = fmadd(wt, B, (C(3) * sw) * eF);
+    } else {
+      even = wt * A;
+      odd = wt * B;
+    }
     const C dq = d[q];
     const C dm = d[m];
     d[q] = fmadd(keep, dq, even + odd);
     d[m] = fmadd(keep, dm, even - odd);
   }
-  if constexpr (FIXUP) {
-    C m0 = 0, ma = 0, mb = 0, mc = 0;
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) {
-      m0 += d[q];
-      if (L::ea[q] > 0) ma += d[q];
-      if (L::ea[q] < 0) ma -= d[q];
-      if (L::eb[q] > 0) mb += d[q];
-      if (L::eb[q] < 0) mb -= d[q];
-      if (L::ec[q] > 0) mc += d[q];
-      if (L::ec[q] < 0) mc -= d[q];
+}
+
+// The direction +1 along internal axis AXIS and 0 along the others
+// (D3Q19: q = 1, 3, 5; D2Q9: q = 1, 3 along b, c), -1 when the lattice has
+// no such axis.
+template <class L>
+__host__ __device__ constexpr int axis_dir(int axis) {
+  for (int q = 1; q < L::Q; ++q) {
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) ok = ok && ecomp<L>(k, q) == (k == axis ? 1 : 0);
+    if (ok) return q;
+  }
+  return -1;
+}
+
+// Collision for storage T with (float compute) an exact conservation step.
+//
+// In exact arithmetic the collision keeps sum d = rho - 1 and moves sum d e
+// from j to j + F. Rounded results miss both: float32 arithmetic by about an
+// ulp per step, and 16-bit storage by the rounding of every stored value
+// (about sqrt(Q) times the rounding of one). Over a run the misses add up in
+// rho and u, so the stored values are made to meet both targets: every
+// population but four is taken as stored (St::quant: the rounding of w + d
+// for float32 f, none for f - w in float32) and summed as its pair is
+// collided, the direction along +a, +b
+// and +c is solved from its axis's momentum target, and the rest population
+// from the mass target, each rounded as stored. The miss per step is then
+// the rounding of one stored value per moment. The float64 kernels (exact
+// enough) skip it.
+template <class L, class T, class C, bool HASF = true>
+__device__ __forceinline__ void collide_for(C (&d)[L::Q], C inv_tau, C Fa, C Fb, C Fc, C drho,
+                                            C rho, C ua, C ub, C uc) {
+  if constexpr (sizeof(C) != 4) {
+    collide_impl<L, C, HASF>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+  } else {
+    using St = Storage<T>;
+    constexpr int qa = axis_dir<L>(0), qb = axis_dir<L>(1), qc = axis_dir<L>(2);
+    static_assert(qb > 0 && qc > 0, "lattice without +b / +c directions");
+    static_assert(L::opp[qb] > qb && L::opp[qc] > qc && (qa < 0 || L::opp[qa] > qa),
+                  "solved directions lead their pairs");
+    // momentum targets: the momentum before the collision (the sums
+    // moments() takes) plus the force
+    C ta = axis_sum<L, 0, -1, C>(d), tb = axis_sum<L, 1, -1, C>(d), tc = axis_sum<L, 2, -1, C>(d);
+    if constexpr (HASF) {
+      ta += Fa;
+      tb += Fb;
+      tc += Fc;
     }
-    // targets: rho - 1 (unchanged) and j + F
-    const C a0 = drho - m0;
-    const C ba = C(3) * ((ja + Fa) - ma);
-    const C bb = C(3) * ((jb + Fb) - mb);
-    const C bc = C(3) * ((jc + Fc) - mc);
+    const C usq15 = C(1.5) * fmadd(ua, ua, fmadd(ub, ub, uc * uc));
+    const C keep = C(1) - inv_tau;
+    const C rho3 = C(3) * rho;
+    const C rho45 = C(4.5) * rho;
+    // sums of the populations taken as stored: mass and e-weighted per axis
+    C s0 = C(0), sa = C(0), sb = C(0), sc = C(0);
+    bool any0 = false, anya = false, anyb = false, anyc = false;
+    auto take = [&](int q, C v) {
+      eterm(1, v, s0, any0);
+      eterm(L::ea[q], v, sa, anya);
+      eterm(L::eb[q], v, sb, anyb);
+      eterm(L::ec[q], v, sc, anyc);
+    };
 #pragma unroll
-    for (int q = 0; q < L::Q; ++q)
-      d[q] = fmadd(weight<L, C>(q), edot_from<L, C>(q, a0, ba, bb, bc), d[q]);
+    for (int q = 1; q < L::Q; ++q) {
+      if (L::opp[q] < q) continue;  // each pair once, from its first member
+      const int m = L::opp[q];
+      const C w = weight<L, C>(q);
+      const C eu = edot<L, C>(q, ua, ub, uc);
+      const C A = fmadd(rho45 * eu, eu, fmadd(-rho, usq15, drho));  // even part of (feq - w)/w
+      const C B = rho3 * eu;                                         // odd part
+      const C wt = inv_tau * w;
+      C even, odd;
+      if constexpr (HASF) {
+        const C uF3 = C(3) * fmadd(ua, Fa, fmadd(ub, Fb, uc * Fc));
+        const C eF = edot<L, C>(q, Fa, Fb, Fc);
+        const C sw = (C(1) - C(0.5) * inv_tau) * w;
+        even = fmadd(wt, A, sw * fmadd(C(9) * eu, eF, -uF3));
+        odd = fmadd(wt, B, (C(3) * sw) * eF);
+      } else {
+        even = wt * A;
+        odd = wt * B;
+      }
+      const bool solved = q == qa || q == qb || q == qc;
+      if (!solved) {
+        d[q] = St::template quant<L>(q, fmadd(keep, d[q], even + odd));
+        take(q, d[q]);
+      }
+      d[m] = St::template quant<L>(m, fmadd(keep, d[m], even - odd));
+      take(m, d[m]);
+    }
+    if constexpr (qa > 0) {
+      d[qa] = St::template quant<L>(qa, ta - sa);
+      s0 += d[qa];
+    }
+    d[qb] = St::template quant<L>(qb, tb - sb);
+    d[qc] = St::template quant<L>(qc, tc - sc);
+    d[0] = St::template quant<L>(0, drho - ((s0 + d[qb]) + d[qc]));
   }
 }
 
-// Collision for storage T: the float32 fix-up where the stored values are
-// float32 (f or f - w), none for float64 (exact enough) or 16-bit storage.
+// The values a store and reload of collide_for's results give back, for
+// kernels that keep them on chip instead (the two-step launches). Where the
+// conservation step already rounds them as stored (kQuantExact), they are
+// those values.
 template <class L, class T, class C>
-__device__ __forceinline__ void collide_for(C (&d)[L::Q], C inv_tau, C Fa, C Fb, C Fc, C drho,
-                                            C rho, C ua, C ub, C uc) {
-  constexpr bool FIXUP = sizeof(C) == 4 && sizeof(T) == 4;
-  collide_impl<L, C, FIXUP>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+__device__ __forceinline__ void as_stored(C (&d)[L::Q]) {
+  using St = Storage<T>;
+  if constexpr (!(St::kQuantExact && sizeof(C) == 4)) {
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+      const T s = St::template enc<L>(q, d[q]);
+      d[q] = St::template load<L>(&s, q);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -144,14 +144,20 @@ __device__ __forceinline__ double lane_from_next(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-template <class L, class T, int V, bool PULL, bool COLLIDE>
+// HASF = false: the launch has no force (uniform zero, no field), so the
+// collision leaves out its exact-zero force terms (same values).
+template <class L, class T, int V, bool PULL, bool COLLIDE, bool HASF = true>
 __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) {
   using C = typename Storage<T>::C;
   using St = Storage<T>;
   constexpr int SEG = 64 * V;
   const int lane = threadIdx.x & 63;
   const int segs = a.nc / SEG;  // wave segments per row
-  const long long wave = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  // the wave's index is the same in every lane: taken from the first lane,
+  // the row arithmetic below stays scalar and each load or store is one
+  // scalar row base plus the lane's offset
+  const long long wave = (long long)blockIdx.x * (blockDim.x / 64) +
+                         __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const long long nrows = (long long)a.na * a.nb;
   if (wave >= nrows * segs) return;  // whole wave leaves together
   const int row = (int)(wave / segs);
@@ -167,6 +173,7 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
   const int c_after = c0 + SEG == a.nc ? 0 : c0 + SEG;
   const long long P = a.plane;
   const T* src = static_cast<const T*>(a.src);
+  const uint32_t lofs = (uint32_t)lane * (uint32_t)(V * sizeof(T));
 
   C d[L::Q][V];
 #pragma unroll
@@ -175,7 +182,8 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
     const int sr = PULL ? (A[1 - L::ea[q]] * a.nb + B[1 - L::eb[q]]) : row;
     const T* rowp = src + q * P + (long long)sr * a.nc;
     T raw[V];
-    vload<T, V>(rowp + cl, raw);
+    // scalar segment base plus the lane's 32-bit byte offset
+    vload<T, V>(reinterpret_cast<const T*>(reinterpret_cast<const char*>(rowp + c0) + lofs), raw);
     C v[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) v[k] = St::template load<L>(&raw[k], q);
@@ -200,7 +208,8 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
     }
   }
 
-  const long long cell0 = (long long)row * a.nc + cl;
+  const long long seg0 = (long long)row * a.nc + c0;  // first cell of the segment
+  const long long cell0 = seg0 + lane * V;
   T out[L::Q][V];
 #pragma unroll
   for (int k = 0; k < V; ++k) {
@@ -208,14 +217,14 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) dk[q] = d[q][k];
     const int cell = (int)(cell0 + k);
-    C Fa, Fb, Fc;
-    load_force<L, C>(a, cell, Fa, Fb, Fc);
+    C Fa = C(0), Fb = C(0), Fc = C(0);
+    if constexpr (HASF) load_force<L, C>(a, cell, Fa, Fb, Fc);
     C drho, rho, ua, ub, uc;
-    moments<L, C>(dk, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+    moments<L, C, HASF>(dk, Fa, Fb, Fc, drho, rho, ua, ub, uc);
     if (a.rho || a.u || a.p) write_moments<L, C>(a, cell, false, rho, ua, ub, uc);
     if (COLLIDE) {
       const C inv_tau = a.tau_field ? recip(C(a.tau_field[cell])) : a.inv_tau;
-      collide_for<L, T, C>(dk, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C, HASF>(dk, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
     }
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
@@ -227,7 +236,9 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
   }
   T* dst = static_cast<T*>(a.dst);
 #pragma unroll
-  for (int q = 0; q < L::Q; ++q) vstore_nt<T, V>(dst + q * P + cell0, out[q]);
+  for (int q = 0; q < L::Q; ++q)
+    vstore_nt<T, V>(reinterpret_cast<T*>(reinterpret_cast<char*>(dst + q * P + seg0) + lofs),
+                    out[q]);
 }
 
 // Zou-He face `which` applied in place to a post-stream state (the push
@@ -452,7 +463,15 @@ int launch_row(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>::
   const long long waves = (long long)g.na * g.nb * (g.nc / (64 * V));
   const long long blocks = (waves + 3) / 4;
   if (blocks >= (1LL << 31)) return fail(LBM_ERR_UNSUPPORTED, "grid too large");
-  hipLaunchKernelGGL((k_row<L, T, V, PULL, COLLIDE>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  // a colliding launch without a force leaves out the force terms (the
+  // template argument is !COLLIDE so that a non-colliding instantiation, which
+  // never takes this branch, is the same kernel as the one below)
+  if (COLLIDE && !a.force_field && a.Fa == 0 && a.Fb == 0 && a.Fc == 0)
+    hipLaunchKernelGGL((k_row<L, T, V, PULL, COLLIDE, !COLLIDE>), dim3((unsigned)blocks),
+                       dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_row<L, T, V, PULL, COLLIDE, true>), dim3((unsigned)blocks), dim3(256),
+                       0, s, a);
   return check_launch("k_row launch");
 }
 
@@ -677,6 +696,26 @@ inline bool pair9_plan(const lbm_desc* d, const Geom& g, bool fields, Pair9Args*
   return true;
 }
 
+// One two-step launch of kind (S1, S2) in the form the call needs.
+template <class T, int S1, int S2>
+int launch_pair_gen(hipStream_t s, const PairArgs& pa) {
+  const dim3 grid((unsigned)pa.nblocks), block(1024);
+  switch (pair_form(pa)) {
+    case PAIR_PLAIN:
+      hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_PLAIN>), grid, block, 0, s, pa);
+      break;
+    case PAIR_FORCE:
+      hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_FORCE>), grid, block, 0, s, pa);
+      break;
+    case PAIR_GEN:
+      hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_GEN>), grid, block, 0, s, pa);
+      break;
+    default:
+      hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_GEN_FACES>), grid, block, 0, s, pa);
+  }
+  return check_launch("k_pair launch");
+}
+
 // The two-step launch of each lattice: its argument block, plan and kernel.
 template <class L> struct PairOf;
 template <> struct PairOf<Lattice<19>> {
@@ -684,9 +723,7 @@ template <> struct PairOf<Lattice<19>> {
   static bool plan(const lbm_desc* d, const Geom& g, bool f, Args* a) { return pair_plan(d, g, f, a); }
   template <class T>
   static int launch(hipStream_t s, const Args& pa) {
-    hipLaunchKernelGGL((k_pair<T, PAIR_GATHER, PAIR_COLLIDE, false>), dim3((unsigned)pa.nblocks),
-                       dim3(1024), 0, s, pa);
-    return check_launch("k_pair launch");
+    return launch_pair_gen<T, PAIR_GATHER, PAIR_COLLIDE>(s, pa);
   }
 };
 template <> struct PairOf<Lattice<9>> {
@@ -698,19 +735,6 @@ template <> struct PairOf<Lattice<9>> {
     return check_launch("k_pair9 launch");
   }
 };
-
-// One two-step launch of kind (S1, S2): the general form where the call has
-// walls, fields or faces, the periodic form otherwise (fewer registers).
-template <class T, int S1, int S2>
-int launch_pair_gen(hipStream_t s, const PairArgs& pa) {
-  if (pa.mask || pa.force_field || pa.tau_field || pa.zh_n)
-    hipLaunchKernelGGL((k_pair<T, S1, S2, true>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s,
-                       pa);
-  else
-    hipLaunchKernelGGL((k_pair<T, S1, S2, false>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s,
-                       pa);
-  return check_launch("k_pair launch");
-}
 
 // The general fields of a two-step launch from the single-step arguments of
 // the same call: mask (reference wall rule only), force and tau fields,

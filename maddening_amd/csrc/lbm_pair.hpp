@@ -110,18 +110,33 @@ struct PairAux {
   float tau;
 };
 
+// Forms of the march, from the leanest: periodic without a force (no force
+// terms in the arithmetic), periodic with a uniform force, the general form
+// (reference walls, tau and force fields), and the general form with Zou-He
+// faces. Each form is its own instantiation, so a form's registers are not
+// raised by what only a later form needs.
+enum { PAIR_PLAIN = 0, PAIR_FORCE = 1, PAIR_GEN = 2, PAIR_GEN_FACES = 3 };
+
 // The march of one workgroup over c-tile ct, b-tile bt and a-chunk `chunk`
 // of lattice a; xx is the workgroup's exchange buffer, two halves of 64 KB. With whole rows
 // (w == nc) the periodic wrap along c stays inside the workgroup; with
 // c-tiles the halo columns compute step 1 only, and step 2 of the tc inner
 // columns reads them.
-template <class T, int S1, int S2, bool GEN>
+template <class T, int S1, int S2, int FORM>
 __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, int chunk,
                                            float (&xx)[2][16][1024]) {
   using L = Lattice<19>;
   using St = Storage<T>;
   using C = float;
   static_assert(sizeof(typename Storage<T>::C) == 4, "float compute only");
+  constexpr bool GEN = FORM >= PAIR_GEN;
+  constexpr bool FACES = FORM == PAIR_GEN_FACES;
+  constexpr bool HASF = FORM != PAIR_PLAIN;
+  // LEAN (the plain form, whose registers allow it): scalar plane bases
+  // with per-thread 32-bit offsets for the gather, issued before step 1's
+  // arithmetic. The other forms keep 64-bit per-lane addresses and gather
+  // after the barrier, which fits them in 128 registers.
+  constexpr bool LEAN = FORM == PAIR_PLAIN;
   const int a0 = chunk * a.la;
   const int a1 = a0 + a.la < a.na ? a0 + a.la : a.na;  // output planes [a0, a1)
   const int b0 = bt * a.tb;
@@ -151,25 +166,65 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
   const int cl = ct * a.tc - a.halo + j;
   const int c = cl < 0 ? cl + nc : (cl >= nc ? cl - nc : cl);
   const bool active = r < a.tb + 2;
-  const int b = (b0 - 1 + r + nb) % nb;
+  // a thread past the workgroup's rows takes the last row's cell: its
+  // (unused) gathers then hit lines the workgroup reads anyway
+  const int b = (b0 - 1 + (active ? r : a.tb + 1) + nb) % nb;
   const bool inner = r >= 1 && r <= a.tb && b0 + r - 1 < nb && j >= a.halo &&
                      j < a.halo + a.tc && cl < nc;
   const T* src = static_cast<const T*>(a.src);
   T* dst = static_cast<T*>(a.dst);
-
-  // step 1's input at plane p: the populations, and for a general launch the
-  // cell's mask byte, force and tau (loaded with them, used after they land)
-  auto fetch = [&](int p, C (&v)[19], PairAux& au) {
-    Nbr n;
-    n.init((p + na) % na, b, c, na, nb, nc);
+  const int plane = nb * nc;
+  // LEAN addresses: plane offsets along a are the same for the whole
+  // workgroup, so each population's base pointer for a plane is uniform
+  // (scalar registers), and a thread adds one of nine 32-bit byte offsets
+  // that stay fixed through the march: off[sb + 1][sc + 1] for the cell
+  // (b + sb, c + sc).
+  uint32_t off[3][3];
+  if constexpr (LEAN) {
+    const int Bs[3] = {b == 0 ? nb - 1 : b - 1, b, b == nb - 1 ? 0 : b + 1};
+    const int Cs[3] = {c == 0 ? nc - 1 : c - 1, c, c == nc - 1 ? 0 : c + 1};
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const int at = S1 == PAIR_LOCAL ? n.cell : n.template at<L>(q, -1);
-      v[q] = St::template load<L>(src + q * P + at, q);
-    }
-    if constexpr (GEN) {
-      au.m = a.mask ? a.mask[n.cell] : 0u;
-      au.tau = a.tau_field ? a.tau_field[n.cell] : 0.0f;
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) off[i][k] = (uint32_t)((Bs[i] * nc + Cs[k]) * (int)sizeof(T));
+  }
+  // population q of plane ia (0 <= ia < na), as a uniform byte pointer
+  auto qplane = [&](const void* base, int q, int ia) {
+    return static_cast<const char*>(base) + ((size_t)q * P + (size_t)ia * plane) * sizeof(T);
+  };
+
+  // step 1's input at plane p (-1 <= p <= na): the populations, and for a
+  // general launch the cell's mask byte and tau (loaded with them, used
+  // after they land)
+  auto fetch = [&](int p, C (&v)[19], PairAux& au) {
+    const int pc = p < 0 ? p + na : (p >= na ? p - na : p);
+    if constexpr (LEAN) {
+      // keep the offsets 32-bit here: hoisted out of the loop, base + offset
+      // would become one 64-bit vector pointer per source, held in registers
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(off[i][k]));
+      const int pl[3] = {pc == 0 ? na - 1 : pc - 1, pc, pc == na - 1 ? 0 : pc + 1};
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        const char* at =
+            S1 == PAIR_LOCAL ? qplane(src, q, pc) + off[1][1]
+                             : qplane(src, q, pl[1 - L::ea[q]]) + off[1 - L::eb[q]][1 - L::ec[q]];
+        v[q] = St::template load<L>(reinterpret_cast<const T*>(at), q);
+      }
+    } else {
+      Nbr n;
+      n.init(pc, b, c, na, nb, nc);
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        const int at = S1 == PAIR_LOCAL ? n.cell : n.template at<L>(q, -1);
+        v[q] = St::template load<L>(src + q * P + at, q);
+      }
+      if constexpr (GEN) {
+        au.m = a.mask ? a.mask[n.cell] : 0u;
+        au.tau = a.tau_field ? a.tau_field[n.cell] : 0.0f;
+      }
     }
   };
   auto inv_tau_of = [&](const PairAux& au) {
@@ -185,6 +240,13 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
       Fc = a.force_field[3 * cell + 2];
     }
   };
+  // after a gather: the reference wall rule at wall cells, faces elsewhere
+  auto walls_faces = [&](C (&v)[19], const PairAux& au, int ia) {
+    if constexpr (GEN) {
+      if (au.m) swap_pairs<L, C>(v);
+      else if constexpr (FACES) apply_faces<L, C>(a, ia, b, c, v);
+    }
+  };
 
   // own column's step-1 values: Z of plane p - 1 (e_a = 0), P1 / P2 of planes
   // p - 1 / p - 2 (e_a = +1); only the entries of those classes are used
@@ -192,26 +254,24 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
   PairAux nau = {0u, 0.0f}, cur = nau, prev = nau;
 #pragma unroll
   for (int q = 0; q < 19; ++q) Z[q] = P1[q] = P2[q] = C(0);
-  if (active) fetch(a0 - 1, nx, nau);
+  if (LEAN || active) fetch(a0 - 1, nx, nau);
   for (int p = a0 - 1; p <= a1; ++p) {
     C d[19];
 #pragma unroll
     for (int q = 0; q < 19; ++q) d[q] = nx[q];
     if constexpr (GEN) cur = nau;
+    // LEAN: the next plane's gather, issued before this plane's arithmetic;
+    // on the last plane a repeat of plane a1, and by every thread (those
+    // past the workgroup's rows load the last row's cells again), so every
+    // plane issues the same loads
+    if constexpr (LEAN) fetch(p < a1 ? p + 1 : a1, nx, nau);
     {
-      if constexpr (GEN && S1 == PAIR_GATHER) {
-        if (cur.m) swap_pairs<L, C>(d);
-        else if (a.zh_n) apply_faces<L, C>(a, (p + na) % na, b, c, d);
-      }
+      if constexpr (S1 == PAIR_GATHER) walls_faces(d, cur, (p + na) % na);
       C drho, rho, ua, ub, uc, Fa, Fb, Fc;
       force_at((((p + na) % na) * nb + b) * nc + c, Fa, Fb, Fc);
-      moments<L, C>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
-      collide_for<L, T, C>(d, inv_tau_of(cur), Fa, Fb, Fc, drho, rho, ua, ub, uc);
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {  // the value a store and reload would give
-        const T s = St::template enc<L>(q, d[q]);
-        d[q] = St::template load<L>(&s, q);
-      }
+      moments<L, C, HASF>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C, HASF>(d, inv_tau_of(cur), Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      as_stored<L, T, C>(d);  // the values a store and reload would give
     }
     const bool out = p > a0;
     // planes alternate between the two halves of the exchange buffer, so the
@@ -224,9 +284,11 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
         x[q - 3][slot] = L::ea[q] < 0 ? d[q] : (L::ea[q] == 0 ? Z[q] : P2[q]);
     }
     __syncthreads();
-    if (p < a1 && active) fetch(p + 1, nx, nau);
-    if (out && inner) {
-      C e[19];
+    if constexpr (!LEAN) {
+      if (p < a1 && active) fetch(p + 1, nx, nau);
+    }
+    // step 2 at plane p - 1: gather g_{n+1} from step 1's results
+    auto gather2 = [&](C (&e)[19]) {
       e[0] = Z[0];   // (p - 1, b, c)
       e[1] = P2[1];  // (p - 2, b, c)
       e[2] = d[2];   // (p, b, c)
@@ -238,14 +300,15 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
         cc = cc < 0 ? cc + w : (cc >= w ? cc - w : cc);
         e[q] = x[q - 3][(r - L::eb[q]) * w + cc];
       }
-      if constexpr (GEN) {
-        if (prev.m) swap_pairs<L, C>(e);
-        else if (a.zh_n) apply_faces<L, C>(a, p - 1, b, c, e);
-      }
+      walls_faces(e, prev, p - 1);
+    };
+    if (out && inner) {
+      C e[19];
+      gather2(e);
       const int cell = ((p - 1) * nb + b) * nc + c;
       C drho, rho, ua, ub, uc, Fa, Fb, Fc;
       force_at(cell, Fa, Fb, Fc);
-      moments<L, C>(e, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      moments<L, C, HASF>(e, Fa, Fb, Fc, drho, rho, ua, ub, uc);
       if constexpr (S2 == PAIR_MOMENTS) {
         // the stream + moments launch's outputs (lbm.py:970-976)
         if (a.rho) a.rho[cell] = rho;
@@ -257,7 +320,7 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
           a.u[3 * cell + 2] = wall ? C(0) : uc;
         }
       } else {
-        collide_for<L, T, C>(e, inv_tau_of(prev), Fa, Fb, Fc, drho, rho, ua, ub, uc);
+        collide_for<L, T, C, HASF>(e, inv_tau_of(prev), Fa, Fb, Fc, drho, rho, ua, ub, uc);
       }
 #pragma unroll
       for (int q = 0; q < 19; ++q) St::template store<L>(dst + q * P + cell, q, e[q]);
@@ -338,11 +401,9 @@ __device__ __forceinline__ void pair_march_ring(const PairArgs& a, int bt, int c
       C drho, rho, ua, ub, uc;
       moments<L, C>(d, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
       collide_for<L, T, C>(d, a.inv_tau, a.Fa, a.Fb, a.Fc, drho, rho, ua, ub, uc);
+      as_stored<L, T, C>(d);  // the values a store and reload would give
 #pragma unroll
-      for (int q = 0; q < 19; ++q) {  // the value a store and reload would give
-        const T s = St::template enc<L>(q, d[q]);
-        x.v[ring_row<L>(q, t)][tid] = St::template load<L>(&s, q);
-      }
+      for (int q = 0; q < 19; ++q) x.v[ring_row<L>(q, t)][tid] = d[q];
     }
     __syncthreads();
     if (p > a0 && inner) {
@@ -367,12 +428,19 @@ __device__ __forceinline__ void pair_march_ring(const PairArgs& a, int bt, int c
   }
 }
 
-template <class T, int S1, int S2, bool GEN>
+template <class T, int S1, int S2, int FORM>
 __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   __shared__ float x[2][16][1024];
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   const int rest = lb / a.ctiles;
-  pair_march<T, S1, S2, GEN>(a, lb - rest * a.ctiles, rest % a.btiles, rest / a.btiles, x);
+  pair_march<T, S1, S2, FORM>(a, lb - rest * a.ctiles, rest % a.btiles, rest / a.btiles, x);
+}
+
+// The form a launch needs (PAIR_PLAIN .. PAIR_GEN_FACES).
+inline int pair_form(const PairArgs& a) {
+  if (a.zh_n) return PAIR_GEN_FACES;
+  if (a.mask || a.force_field || a.tau_field) return PAIR_GEN;
+  return a.Fa != 0.0f || a.Fb != 0.0f || a.Fc != 0.0f ? PAIR_FORCE : PAIR_PLAIN;
 }
 
 // ---------------------------------------------------------------------------
@@ -439,11 +507,7 @@ __global__ void __launch_bounds__(1024) k_pair9(Pair9Args a) {
       C drho, rho, ua, ub, uc;
       moments<L, C>(d, C(0), a.Fb, a.Fc, drho, rho, ua, ub, uc);
       collide_for<L, T, C>(d, a.inv_tau, C(0), a.Fb, a.Fc, drho, rho, ua, ub, uc);
-#pragma unroll
-      for (int q = 0; q < 9; ++q) {  // the value a store and reload would give
-        const T s = St::template enc<L>(q, d[q]);
-        d[q] = St::template load<L>(&s, q);
-      }
+      as_stored<L, T, C>(d);  // the values a store and reload would give
     }
     const bool out = p > b0;
     if (out) {

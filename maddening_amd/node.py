@@ -144,6 +144,12 @@ class LBMNode(SimulationNode):
         else:
             self._has_walls = False
             self._wall_mask = torch.zeros(self._grid_shape, dtype=torch.uint8, device=self._device)
+        # a mask without walls runs the periodic kernels: the wall rule then
+        # changes no cell, so the values are the same, and the periodic
+        # two-step form is the leanest
+        self._open_kernel = None
+        self._mask_seen = None  # (mask tensor, (data_ptr, version)) last checked
+        self._mask_walls = True
 
     # -- properties (lbm.py:690-705) ---------------------------------------
     @property
@@ -223,6 +229,23 @@ class LBMNode(SimulationNode):
                      None if p_out is None else (self._outlet_face, num(p_out) / self._cs2))
         return mask, force, faces
 
+    def _kernel_for(self, mask: torch.Tensor):
+        """(kernel, mask argument) for a call on ``mask``. Whether the mask
+        has a wall is read from the device once per mask tensor and version
+        (one synchronisation when the mask changes, none per step)."""
+        key = (mask.data_ptr(), mask._version)
+        if self._mask_seen is None or self._mask_seen[0] is not mask or self._mask_seen[1] != key:
+            self._mask_walls = bool(mask.any())
+            self._mask_seen = (mask, key)
+        if self._mask_walls:
+            return self._kernel, mask.to(torch.uint8)
+        if self._open_kernel is None:
+            k = self._kernel
+            self._open_kernel = LBMKernel(self._grid_shape, lattice=self._lat.name, tau=self._tau,
+                                          storage=k.storage, bb="none", scheme=k.scheme,
+                                          pair=k.pair, device=self._device)
+        return self._open_kernel, None
+
     def update(self, state: dict, boundary_inputs: dict, dt: float) -> dict:
         """One LBM step (lbm.py:899-986)."""
         return self.run(state, boundary_inputs, 1)
@@ -237,7 +260,7 @@ class LBMNode(SimulationNode):
         the HIP adjoint (sqrt(N) checkpointing). That is the counterpart of
         jax.grad through the reference's update."""
         mask, force, faces = self._inputs(state, boundary_inputs)
-        k = self._kernel
+        k, kmask = self._kernel_for(mask)
         raw_force = boundary_inputs.get("body_force")
         raw_p = (boundary_inputs.get("inlet_pressure"), boundary_inputs.get("outlet_pressure"))
         wants_grad = torch.is_grad_enabled() and (
@@ -261,10 +284,10 @@ class LBMNode(SimulationNode):
                 # reference); the cast passes gradients through unchanged
                 gforce = gforce.to(self._device).float()
             f_new, rho, u, p = rollout(k, self._soa(state["f"]), nsteps, force=gforce,
-                                       mask=mask.to(torch.uint8), faces=gfaces)
+                                       mask=kmask, faces=gfaces)
         else:
-            f_new, rho, u, p = k.step(self._soa(state["f"]), nsteps, mask=mask.to(torch.uint8),
-                                      force=force, faces=faces)
+            f_new, rho, u, p = k.step(self._soa(state["f"]), nsteps, mask=kmask, force=force,
+                                      faces=faces)
         result = {
             "f": k.aos_view(f_new),
             "density": rho.to(self._dtype),
