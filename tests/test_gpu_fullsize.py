@@ -7,9 +7,9 @@ itself is pinned by tests/test_oracle.py and the golden vectors.
 * D3Q19 256^3 Taylor-Green (u0 = 0.05, tau = 0.8), 1000 steps: float32 and
   float32-deviation storage within 1e-5 relative of float64 on rho and u.
 * The same case with float16 deviation storage at steps 100, 1000 and 5000:
-  max |du| / u0 below 1e-3 through step 1000 (measured 4.0e-4, 9.3e-4) and
-  below the achieved 1.3e-3 at step 5000 (measured 1.23e-3); bfloat16
-  deviations below 2e-2, 5e-2, 1e-1.
+  max |du| / u0 below 1e-3 at every checkpoint (measured 4.6e-4, 4.1e-4,
+  2.5e-4); bfloat16 deviations below 1e-2 (measured 4.4e-3, 3.1e-3,
+  2.2e-3).
 * D3Q19 128x128x256 sphere-packed channel (porosity 0.7, tau field 0.7,
   F = (0, 0, 1e-5)), 500 steps, loss = mean u_z over the outlet plane:
   the sqrt(N)-checkpointed float64 adjoint against central differences,
@@ -82,13 +82,12 @@ def test_config2_256_float32_within_1e5_after_1000_steps(tgv256_reference, stora
 
 
 # max |du| / u0 against the float64 kernel at steps 100, 1000, 5000 (the
-# workload's checkpoints). float16 deviations meet 1e-3 through step 1000;
-# at step 5000 the bound asserted is the one achieved, 1.3e-3 (measured
-# 1.23e-3): quantisation of the stored deviations adds an unbiased rounding
-# error every step, and its random walk in the conserved momentum keeps
-# growing after the flow itself has decayed to about half of u0.
-DEV_BOUNDS = {"f16dev": {100: 1e-3, 1000: 1e-3, 5000: 1.3e-3},
-              "bf16dev": {100: 2e-2, 1000: 5e-2, 5000: 1e-1}}
+# workload's checkpoints). The collision's conservation step makes the
+# stored 16-bit values keep mass and momentum exactly but for the rounding
+# of one stored value per moment, so the rounding noise no longer random-
+# walks in the conserved moments and the error falls as the flow decays.
+DEV_BOUNDS = {"f16dev": {100: 1e-3, 1000: 1e-3, 5000: 1e-3},
+              "bf16dev": {100: 1e-2, 1000: 1e-2, 5000: 1e-2}}
 
 
 @pytest.mark.parametrize("storage", ["f16dev", "bf16dev"])
