@@ -124,6 +124,7 @@ constexpr float kHalfScale = 1024.0f;
 template <> struct Storage<_Float16> {
   using C = float;
   static constexpr bool kQuantExact = true;
+  static constexpr int kLog2Scale = 10;  // stored value = deviation * 2^10
   template <class L>
   static __device__ __forceinline__ float load(const _Float16* p, int) {
     return (float)(*p) * (1.0f / kHalfScale);
@@ -134,6 +135,17 @@ template <> struct Storage<_Float16> {
   template <class L>
   static __device__ __forceinline__ float quant(int, float d) {
     return (float)(_Float16)(d * kHalfScale) * (1.0f / kHalfScale);
+  }
+  // the same three on deviations kept in stored units (d * 2^10). quant_s
+  // rounds D to float32 first, as quant does (it rounds d, then scales it
+  // exactly): left to itself the compiler fuses the fma or subtraction that
+  // made D with the conversion (v_fma_mixlo_f16), rounding once, straight to
+  // half, and the result could differ from quant's by an ulp.
+  static __device__ __forceinline__ float load_s(const _Float16* p) { return (float)(*p); }
+  static __device__ __forceinline__ _Float16 enc_s(float D) { return (_Float16)D; }
+  static __device__ __forceinline__ float quant_s(float D) {
+    asm volatile("" : "+v"(D));
+    return (float)(_Float16)D;
   }
   static __device__ __forceinline__ void put(_Float16* p, _Float16 v) {
     __builtin_nontemporal_store(v, p);
@@ -157,6 +169,48 @@ template <> struct Storage<__bf16> {
   template <class L>
   static __device__ __forceinline__ void store(__bf16* p, int q, float d) { put(p, enc<L>(q, d)); }
 };
+
+// ---------------------------------------------------------------------------
+// Stored units. Float16 storage holds deviations times 2^10; a kernel may
+// keep its deviations in those units (D = d * 2^kLog2Scale) so that loads,
+// the rounding to the stored format and stores need no multiply. Every
+// product and sum of the collision is then the exact scaling of its value in
+// deviation units (a power of two, far from float32's range limits), so the
+// stored bits are the same either way. Storage types without a scale use
+// their deviation units (kLog2Scale 0).
+// ---------------------------------------------------------------------------
+template <class T, class = void>
+struct ScaleOf {
+  static constexpr int v = 0;
+};
+template <class T>
+struct ScaleOf<T, decltype((void)Storage<T>::kLog2Scale)> {
+  static constexpr int v = Storage<T>::kLog2Scale;
+};
+
+// 2^k for 0 <= k < 31, as a compile-time float
+template <class C, int K>
+__host__ __device__ constexpr C pow2() {
+  return C(1u << K);
+}
+
+// load / rounding to the stored format / encode in stored units (SCALED) or
+// deviation units
+template <class L, class T, bool SCALED>
+__device__ __forceinline__ typename Storage<T>::C load_u(const T* p, int q) {
+  if constexpr (SCALED && ScaleOf<T>::v != 0) return Storage<T>::load_s(p);
+  else return Storage<T>::template load<L>(p, q);
+}
+template <class L, class T, bool SCALED, class C>
+__device__ __forceinline__ C quant_u(int q, C v) {
+  if constexpr (SCALED && ScaleOf<T>::v != 0) return Storage<T>::quant_s(v);
+  else return Storage<T>::template quant<L>(q, v);
+}
+template <class L, class T, bool SCALED, class C>
+__device__ __forceinline__ T enc_u(int q, C v) {
+  if constexpr (SCALED && ScaleOf<T>::v != 0) return Storage<T>::enc_s(v);
+  else return Storage<T>::template enc<L>(q, v);
+}
 
 // e_q . (a, b, c) for a lattice vector (components -1, 0 or +1), optionally
 // added to `init`. Terms with a zero component are left out and +-1
@@ -240,13 +294,17 @@ __device__ __forceinline__ C axis_sum(const C (&d)[L::Q]) {
 // term is then an exact zero, so leaving them out changes no value (at most
 // the sign of a zero result), and a kernel that knows it has no force saves
 // their arithmetic and registers.
-template <class L, class C, bool HASF = true>
+//
+// LOG2S > 0: d holds deviations in stored units (times 2^LOG2S); the sums
+// are scaled back exactly, so the moments are the same bits.
+template <class L, class C, bool HASF = true, int LOG2S = 0>
 __device__ __forceinline__ void moments(const C (&d)[L::Q], C Fa, C Fb, C Fc,
                                         C& drho, C& rho, C& ua, C& ub, C& uc) {
-  const C s = mass_sum<L, C>(d);
-  const C ja = axis_sum<L, 0, -1, C>(d);
-  const C jb = axis_sum<L, 1, -1, C>(d);
-  const C jc = axis_sum<L, 2, -1, C>(d);
+  constexpr C un = C(1) / pow2<C, LOG2S>();
+  const C s = LOG2S ? mass_sum<L, C>(d) * un : mass_sum<L, C>(d);
+  const C ja = LOG2S ? axis_sum<L, 0, -1, C>(d) * un : axis_sum<L, 0, -1, C>(d);
+  const C jb = LOG2S ? axis_sum<L, 1, -1, C>(d) * un : axis_sum<L, 1, -1, C>(d);
+  const C jc = LOG2S ? axis_sum<L, 2, -1, C>(d) * un : axis_sum<L, 2, -1, C>(d);
   drho = s;
   rho = C(1) + s;
   const C inv = recip(rho > C(1e-10) ? rho : C(1e-10));
@@ -359,13 +417,18 @@ __host__ __device__ constexpr int axis_dir(int axis) {
 // from the mass target, each rounded as stored. The miss per step is then
 // the rounding of one stored value per moment. The float64 kernels (exact
 // enough) skip it.
-template <class L, class T, class C, bool HASF = true>
+//
+// SCALED: d holds deviations in stored units (ScaleOf<T>); the equilibrium,
+// source and force terms are scaled by the same power of two where they meet
+// d, so every result is the exact scaling of the unscaled one.
+template <class L, class T, class C, bool HASF = true, bool SCALED = false>
 __device__ __forceinline__ void collide_for(C (&d)[L::Q], C inv_tau, C Fa, C Fb, C Fc, C drho,
                                             C rho, C ua, C ub, C uc) {
   if constexpr (sizeof(C) != 4) {
     collide_impl<L, C, HASF>(d, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
   } else {
-    using St = Storage<T>;
+    constexpr int LS = SCALED ? ScaleOf<T>::v : 0;
+    constexpr C S = pow2<C, LS>();
     constexpr int qa = axis_dir<L>(0), qb = axis_dir<L>(1), qc = axis_dir<L>(2);
     static_assert(qb > 0 && qc > 0, "lattice without +b / +c directions");
     static_assert(L::opp[qb] > qb && L::opp[qc] > qc && (qa < 0 || L::opp[qa] > qa),
@@ -374,9 +437,9 @@ __device__ __forceinline__ void collide_for(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
     // moments() takes) plus the force
     C ta = axis_sum<L, 0, -1, C>(d), tb = axis_sum<L, 1, -1, C>(d), tc = axis_sum<L, 2, -1, C>(d);
     if constexpr (HASF) {
-      ta += Fa;
-      tb += Fb;
-      tc += Fc;
+      ta += LS ? Fa * S : Fa;
+      tb += LS ? Fb * S : Fb;
+      tc += LS ? Fc * S : Fc;
     }
     const C usq15 = C(1.5) * fmadd(ua, ua, fmadd(ub, ub, uc * uc));
     const C keep = C(1) - inv_tau;
@@ -399,12 +462,13 @@ __device__ __forceinline__ void collide_for(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
       const C eu = edot<L, C>(q, ua, ub, uc);
       const C A = fmadd(rho45 * eu, eu, fmadd(-rho, usq15, drho));  // even part of (feq - w)/w
       const C B = rho3 * eu;                                         // odd part
-      const C wt = inv_tau * w;
+      // wt and sw carry the scale (exact): even and odd come out in d's units
+      const C wt = LS ? (inv_tau * w) * S : inv_tau * w;
       C even, odd;
       if constexpr (HASF) {
         const C uF3 = C(3) * fmadd(ua, Fa, fmadd(ub, Fb, uc * Fc));
         const C eF = edot<L, C>(q, Fa, Fb, Fc);
-        const C sw = (C(1) - C(0.5) * inv_tau) * w;
+        const C sw = LS ? ((C(1) - C(0.5) * inv_tau) * w) * S : (C(1) - C(0.5) * inv_tau) * w;
         even = fmadd(wt, A, sw * fmadd(C(9) * eu, eF, -uF3));
         odd = fmadd(wt, B, (C(3) * sw) * eF);
       } else {
@@ -413,19 +477,19 @@ __device__ __forceinline__ void collide_for(C (&d)[L::Q], C inv_tau, C Fa, C Fb,
       }
       const bool solved = q == qa || q == qb || q == qc;
       if (!solved) {
-        d[q] = St::template quant<L>(q, fmadd(keep, d[q], even + odd));
+        d[q] = quant_u<L, T, SCALED>(q, fmadd(keep, d[q], even + odd));
         take(q, d[q]);
       }
-      d[m] = St::template quant<L>(m, fmadd(keep, d[m], even - odd));
+      d[m] = quant_u<L, T, SCALED>(m, fmadd(keep, d[m], even - odd));
       take(m, d[m]);
     }
     if constexpr (qa > 0) {
-      d[qa] = St::template quant<L>(qa, ta - sa);
+      d[qa] = quant_u<L, T, SCALED>(qa, ta - sa);
       s0 += d[qa];
     }
-    d[qb] = St::template quant<L>(qb, tb - sb);
-    d[qc] = St::template quant<L>(qc, tc - sc);
-    d[0] = St::template quant<L>(0, drho - ((s0 + d[qb]) + d[qc]));
+    d[qb] = quant_u<L, T, SCALED>(qb, tb - sb);
+    d[qc] = quant_u<L, T, SCALED>(qc, tc - sc);
+    d[0] = quant_u<L, T, SCALED>(0, (LS ? drho * S : drho) - ((s0 + d[qb]) + d[qc]));
   }
 }
 

@@ -149,7 +149,9 @@ __device__ __forceinline__ double lane_from_next(double x) {
 template <class L, class T, int V, bool PULL, bool COLLIDE, bool HASF = true>
 __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) {
   using C = typename Storage<T>::C;
-  using St = Storage<T>;
+  // float16 storage: deviations in stored units (exact, fewer multiplies)
+  constexpr bool SCALED = ScaleOf<T>::v != 0;
+  constexpr int LS = ScaleOf<T>::v;
   constexpr int SEG = 64 * V;
   const int lane = threadIdx.x & 63;
   const int segs = a.nc / SEG;  // wave segments per row
@@ -186,21 +188,21 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
     vload<T, V>(reinterpret_cast<const T*>(reinterpret_cast<const char*>(rowp + c0) + lofs), raw);
     C v[V];
 #pragma unroll
-    for (int k = 0; k < V; ++k) v[k] = St::template load<L>(&raw[k], q);
+    for (int k = 0; k < V; ++k) v[k] = load_u<L, T, SCALED>(&raw[k], q);
     if (!PULL || L::ec[q] == 0) {
 #pragma unroll
       for (int k = 0; k < V; ++k) d[q][k] = v[k];
     } else if (L::ec[q] > 0) {
       // cell c takes c - 1: lane's first cell takes the previous lane's last;
       // lane 0 takes the cell before the segment (a wave-uniform load, no branch)
-      const C before = St::template load<L>(rowp + c_before, q);
+      const C before = load_u<L, T, SCALED>(rowp + c_before, q);
       const C edge = lane_from_prev(v[V - 1]);
       d[q][0] = lane == 0 ? before : edge;
 #pragma unroll
       for (int k = 1; k < V; ++k) d[q][k] = v[k - 1];
     } else {
       // cell c takes c + 1: lane's last cell takes the next lane's first
-      const C after = St::template load<L>(rowp + c_after, q);
+      const C after = load_u<L, T, SCALED>(rowp + c_after, q);
       const C edge = lane_from_next(v[0]);
 #pragma unroll
       for (int k = 0; k < V - 1; ++k) d[q][k] = v[k + 1];
@@ -220,18 +222,16 @@ __global__ void __launch_bounds__(256) k_row(FwdArgs<typename Storage<T>::C> a) 
     C Fa = C(0), Fb = C(0), Fc = C(0);
     if constexpr (HASF) load_force<L, C>(a, cell, Fa, Fb, Fc);
     C drho, rho, ua, ub, uc;
-    moments<L, C, HASF>(dk, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+    moments<L, C, HASF, LS>(dk, Fa, Fb, Fc, drho, rho, ua, ub, uc);
     if (a.rho || a.u || a.p) write_moments<L, C>(a, cell, false, rho, ua, ub, uc);
     if (COLLIDE) {
       const C inv_tau = a.tau_field ? recip(C(a.tau_field[cell])) : a.inv_tau;
-      collide_for<L, T, C, HASF>(dk, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C, HASF, SCALED>(dk, inv_tau, Fa, Fb, Fc, drho, rho, ua, ub, uc);
     }
 #pragma unroll
     for (int q = 0; q < L::Q; ++q) {
       // same conversion as Storage<T>::store, kept in registers for a vector store
-      if constexpr (sizeof(T) >= 4) out[q][k] = T(weight<L, C>(q) + dk[q]);
-      else if constexpr (std::is_same<T, _Float16>::value) out[q][k] = T(dk[q] * kHalfScale);
-      else out[q][k] = T(dk[q]);
+      out[q][k] = enc_u<L, T, SCALED>(q, dk[q]);
     }
   }
   T* dst = static_cast<T*>(a.dst);
@@ -510,15 +510,13 @@ int launch_push(hipStream_t s, const Geom& g, const FwdArgs<typename Storage<T>:
 // Two steps per launch (k_pair, lbm_pair.hpp)
 // ---------------------------------------------------------------------------
 
-// Storage types the two-step launches are built for: 32-bit storage, and
-// for D2Q9 16-bit deviation storage too. D3Q19 with 16-bit storage measured
-// slower than the row kernel (256^3: 569 us per two steps against 2 x 215
-// us), because k_pair is bound by load latency rather than by bytes; there
-// only a single update in one launch (pair_one) uses the march.
+// Storage types the two-step launches are built for: 32-bit and 16-bit
+// deviation storage (which lattices and forms use them: pair_plan,
+// pair9_plan).
 template <class L, class T>
 constexpr bool pair_storage() {
   return std::is_same<T, float>::value || std::is_same<T, f32dev_t>::value ||
-         (L::Q == 9 && (std::is_same<T, _Float16>::value || std::is_same<T, __bf16>::value));
+         std::is_same<T, _Float16>::value || std::is_same<T, __bf16>::value;
 }
 // Storage types of the one-launch update (k_pair<T, PAIR_LOCAL, PAIR_MOMENTS>).
 template <class L, class T>
@@ -623,12 +621,23 @@ inline bool pair_applies(const lbm_desc* d, const Geom& g, PairArgs* pa) {
 // rule, force or tau fields (`fields`) or pressure faces the march runs its
 // general form under the same rule: on the 128x128x256 sphere channel
 // (30 % walls, tau field) 0.090 against 0.120 ms per step.
+//
+// 16-bit storage: without LBM_FLAG_PAIR_FORCE the march runs only in its
+// plain form (periodic, no force, no fields, no faces: its other forms
+// spill registers with 16-bit storage), under the same shape rule. Measured
+// against the row kernel (MLUPS, march / row): 256^3 f16dev 88.0k / 73.1k,
+// bf16dev 89.9k / 73.4k; 128x128x1024 (tiles) f16dev 82.5k / 74.6k,
+// bf16dev 76.1k / 74.3k; other shapes in the design notes.
 inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
-  (void)fields;
   if (d->flags & LBM_FLAG_PAIR_OFF) return false;
-  if (d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
+  if (d->storage == LBM_F64) return false;
   if (!pair_applies(d, g, pa)) return false;
   if (d->flags & LBM_FLAG_PAIR_FORCE) return true;
+  const bool wide = d->storage == LBM_F32 || d->storage == LBM_F32_DEV;
+  if (!wide && (fields || d->bb_mode != LBM_BB_NONE || d->inlet_face > 0 ||
+                d->outlet_face > 0 || d->force[0] != 0.0 || d->force[1] != 0.0 ||
+                d->force[2] != 0.0))
+    return false;
   if (g.nc > kPairWholeRows) {
     // 64-column tiles: only rows the tiles cover with at most 10 % of extra
     // columns (the measured ragged row, 1000 cells, computes 1024; 342 cells

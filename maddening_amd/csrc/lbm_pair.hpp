@@ -137,6 +137,11 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
   // arithmetic. The other forms keep 64-bit per-lane addresses and gather
   // after the barrier, which fits them in 128 registers.
   constexpr bool LEAN = FORM == PAIR_PLAIN;
+  // float16 storage: deviations in stored units (exact; no multiply per
+  // load, rounding or store), except in the faces form, whose face map
+  // works on deviations
+  constexpr bool SCALED = ScaleOf<T>::v != 0 && !FACES;
+  constexpr int LS = SCALED ? ScaleOf<T>::v : 0;
   const int a0 = chunk * a.la;
   const int a1 = a0 + a.la < a.na ? a0 + a.la : a.na;  // output planes [a0, a1)
   const int b0 = bt * a.tb;
@@ -211,7 +216,7 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
         const char* at =
             S1 == PAIR_LOCAL ? qplane(src, q, pc) + off[1][1]
                              : qplane(src, q, pl[1 - L::ea[q]]) + off[1 - L::eb[q]][1 - L::ec[q]];
-        v[q] = St::template load<L>(reinterpret_cast<const T*>(at), q);
+        v[q] = load_u<L, T, SCALED>(reinterpret_cast<const T*>(at), q);
       }
     } else {
       Nbr n;
@@ -219,7 +224,7 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
 #pragma unroll
       for (int q = 0; q < 19; ++q) {
         const int at = S1 == PAIR_LOCAL ? n.cell : n.template at<L>(q, -1);
-        v[q] = St::template load<L>(src + q * P + at, q);
+        v[q] = load_u<L, T, SCALED>(src + q * P + at, q);
       }
       if constexpr (GEN) {
         au.m = a.mask ? a.mask[n.cell] : 0u;
@@ -269,8 +274,8 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
       if constexpr (S1 == PAIR_GATHER) walls_faces(d, cur, (p + na) % na);
       C drho, rho, ua, ub, uc, Fa, Fb, Fc;
       force_at((((p + na) % na) * nb + b) * nc + c, Fa, Fb, Fc);
-      moments<L, C, HASF>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
-      collide_for<L, T, C, HASF>(d, inv_tau_of(cur), Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      moments<L, C, HASF, LS>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      collide_for<L, T, C, HASF, SCALED>(d, inv_tau_of(cur), Fa, Fb, Fc, drho, rho, ua, ub, uc);
       as_stored<L, T, C>(d);  // the values a store and reload would give
     }
     const bool out = p > a0;
@@ -308,7 +313,7 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
       const int cell = ((p - 1) * nb + b) * nc + c;
       C drho, rho, ua, ub, uc, Fa, Fb, Fc;
       force_at(cell, Fa, Fb, Fc);
-      moments<L, C, HASF>(e, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+      moments<L, C, HASF, LS>(e, Fa, Fb, Fc, drho, rho, ua, ub, uc);
       if constexpr (S2 == PAIR_MOMENTS) {
         // the stream + moments launch's outputs (lbm.py:970-976)
         if (a.rho) a.rho[cell] = rho;
@@ -320,10 +325,10 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
           a.u[3 * cell + 2] = wall ? C(0) : uc;
         }
       } else {
-        collide_for<L, T, C, HASF>(e, inv_tau_of(prev), Fa, Fb, Fc, drho, rho, ua, ub, uc);
+        collide_for<L, T, C, HASF, SCALED>(e, inv_tau_of(prev), Fa, Fb, Fc, drho, rho, ua, ub, uc);
       }
 #pragma unroll
-      for (int q = 0; q < 19; ++q) St::template store<L>(dst + q * P + cell, q, e[q]);
+      for (int q = 0; q < 19; ++q) St::put(dst + q * P + cell, enc_u<L, T, SCALED>(q, e[q]));
     }
     if constexpr (GEN) prev = cur;
 #pragma unroll

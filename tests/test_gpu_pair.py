@@ -43,7 +43,7 @@ def _run(shape, storage, pair, nsteps, seed=5, force=(1e-5, -2e-5, 3e-5)):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("storage", ["f32", "f32dev"])
+@pytest.mark.parametrize("storage", ["f32", "f32dev", "f16dev", "bf16dev"])
 @pytest.mark.parametrize("nsteps", [1, 2, 3, 4, 6])
 def test_pair_launches_equal_single_launches_bitwise(shape, storage, nsteps):
     print(f"pair vs single: {shape} {storage} nsteps={nsteps}", flush=True)
@@ -53,6 +53,40 @@ def test_pair_launches_equal_single_launches_bitwise(shape, storage, nsteps):
     assert pairs == (nsteps + 1) // 2 and launches == (nsteps + 2) // 2
     _, single = _run(shape, storage, "off", nsteps)
     for a, b in zip(forced, single):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("shape", [(64, 80, 128), (33, 50, 64), (2, 2, 3), (5, 14, 512)])
+@pytest.mark.parametrize("storage", ["f16dev", "bf16dev"])
+@pytest.mark.parametrize("nsteps", [2, 5])
+def test_16_bit_plain_form_equals_single_launches_bitwise(shape, storage, nsteps):
+    """16-bit storage without a force: the plain form of the march, whose
+    float16 deviations stay in stored units (2^10 d) through the launch."""
+    print(f"16-bit plain pair vs single: {shape} {storage} nsteps={nsteps}", flush=True)
+    k, forced = _run(shape, storage, "force", nsteps, force=(0.0, 0.0, 0.0))
+    assert k.schedule(nsteps)[1] == (nsteps + 1) // 2
+    _, single = _run(shape, storage, "off", nsteps, force=(0.0, 0.0, 0.0))
+    for a, b in zip(forced, single):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("storage", ["f16dev", "bf16dev"])
+def test_16_bit_auto_pairs_equal_single_launches(storage):
+    """At 128^3 (the rule's smallest size) 16-bit storage takes two-step
+    launches by itself; 10 steps equal one launch per step bit for bit."""
+    from maddening_amd.lbm import LBMKernel
+    from maddening_amd import workloads
+    shape = (128, 128, 128)
+    rho, u = workloads.taylor_green_3d(128, 0.05)
+    out = {}
+    for pair in ("auto", "off"):
+        k = LBMKernel(shape, lattice="D3Q19", tau=0.8, storage=storage, pair=pair)
+        if pair == "auto":
+            assert k.schedule(10) == (6, 5)
+        f = k.equilibrium(torch.from_numpy(rho).float(), torch.from_numpy(u).float())
+        out[pair] = [t.cpu() for t in k.step(f, 10)]
+        torch.cuda.synchronize()
+    for a, b in zip(out["auto"], out["off"]):
         assert torch.equal(a, b)
 
 

@@ -10,7 +10,7 @@ import ctypes
 import pytest
 
 
-def _desc(dims, storage=4, bb=0, flags=0, faces=False, q=19):
+def _desc(dims, storage=4, bb=0, flags=0, faces=False, q=19, force=None):
     from maddening_amd import _lib
     d = _lib.LbmDesc()
     d.q, d.storage, d.bb_mode, d.tau, d.flags = q, storage, bb, 0.8, flags
@@ -18,6 +18,8 @@ def _desc(dims, storage=4, bb=0, flags=0, faces=False, q=19):
         d.dims[i] = n
     if faces:
         d.inlet_face, d.inlet_rho = 5, 1.0
+    for i, v in enumerate(force or ()):
+        d.force[i] = v
     return d
 
 
@@ -55,13 +57,11 @@ def test_auto_pairs_on_measured_winning_shapes(dims):
 
 def test_auto_keeps_single_launches_where_pairs_measured_slower():
     from maddening_amd import _lib
-    # 16-bit and 64-bit storage; fewer than 2^21 cells (96^3 and 100^3
-    # measured slower with two-step launches); rows not a multiple of 32
-    # (200x200x100 broke even), shorter than 64 or longer than 256 cells;
-    # a or b thinner than 64 cells; halfway bounce-back, the push scheme,
-    # PAIR_OFF
-    cases = [_desc((256, 256, 256), storage=2), _desc((256, 256, 256), storage=3),
-             _desc((256, 256, 256), storage=1), _desc((96, 96, 96)), _desc((100, 100, 100)),
+    # 64-bit storage; fewer than 2^21 cells (96^3 and 100^3 measured slower
+    # with two-step launches); rows not a multiple of 32 (200x200x100 broke
+    # even), shorter than 64 or longer than 256 cells; a or b thinner than
+    # 64 cells; halfway bounce-back, the push scheme, PAIR_OFF
+    cases = [_desc((256, 256, 256), storage=1), _desc((96, 96, 96)), _desc((100, 100, 100)),
              _desc((127, 128, 128)), _desc((16, 16, 256)), _desc((200, 200, 100)),
              _desc((256, 256, 100)), _desc((1024, 64, 32)), _desc((256, 128, 288)),
              _desc((256, 256, 320)), _desc((32768, 1, 64)), _desc((2, 4096, 256)),
@@ -89,20 +89,37 @@ def test_force_flag_reaches_every_row_length():
     assert _schedule(_desc((8, 8, 64), flags=F), 9, with_fields=1) == (5, 5)
     # never under halfway bounce-back
     assert _schedule(_desc((8, 8, 64), bb=2, flags=F), 9) == (10, 0)
-    # not 16-bit or 64-bit storage
+    # 16-bit storage in every form when forced; never 64-bit
     for d in [_desc((8, 8, 64), storage=2, flags=F), _desc((8, 8, 512), storage=3, flags=F),
-              _desc((8, 8, 64), storage=1, flags=F)]:
-        assert _schedule(d, 9) == (10, 0)
+              _desc((8, 8, 64), storage=2, bb=1, flags=F),
+              _desc((8, 8, 64), storage=3, faces=True, flags=F)]:
+        assert _schedule(d, 9) == (5, 5)
+    assert _schedule(_desc((8, 8, 64), storage=1, flags=F), 9) == (10, 0)
+
+
+def test_16_bit_auto_pairs_only_in_the_plain_form():
+    """16-bit deviation storage takes two-step launches by itself only where
+    the march runs its plain form (periodic, no force, no fields, no
+    faces) on a shape of the rule; its other forms spill registers."""
+    for storage in (2, 3):  # f16dev, bf16dev
+        for dims in [(256, 256, 256), (512, 256, 128), (256, 256, 160), (128, 128, 1024)]:
+            assert _schedule(_desc(dims, storage=storage), 9) == (5, 5)
+        for d in [_desc((256, 256, 256), storage=storage, bb=1),
+                  _desc((256, 256, 256), storage=storage, faces=True),
+                  _desc((256, 256, 256), storage=storage, force=(0.0, 0.0, 1e-5)),
+                  _desc((96, 96, 96), storage=storage)]:
+            assert _schedule(d, 9) == (10, 0)
+        assert _schedule(_desc((256, 256, 256), storage=storage), 9, with_fields=1) == (10, 0)
 
 
 def test_phase_3_rejected_where_pairs_do_not_apply():
     from maddening_amd import _lib
     L = _lib.lib()
-    d = _desc((8, 8, 64), storage=2, flags=_lib.LBM_FLAG_PAIR_FORCE)
+    d = _desc((8, 8, 64), storage=1, flags=_lib.LBM_FLAG_PAIR_FORCE)
     rc = L.lbm_pull_phase(None, ctypes.byref(d), 3, ctypes.c_void_p(4096), ctypes.c_void_p(8192),
                           None, None, None, None, None, None, None)
     assert rc == -2 and b"two steps per launch" in L.lbm_last_error()
-    for phase in (3, 4, 5):   # sequences: 32-bit storage only
+    for phase in (3, 4, 5):   # sequences: 32- and 16-bit storage only
         assert L.lbm_pull_phase(None, ctypes.byref(d), phase, ctypes.c_void_p(4096),
                                 ctypes.c_void_p(8192), None, None, None, None, None, None,
                                 None) == -2
