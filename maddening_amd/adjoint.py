@@ -73,10 +73,17 @@ def _links(L, st, d, kern, m):
     return links
 
 
-def _two_step(kern: LBMKernel, fields: bool) -> bool:
-    """Whether lbm_step runs this lattice's sequences two units per launch
-    (the same choice, so the rollout's forward launches match its own)."""
-    return kern.D == 3 and kern.schedule(2, with_fields=fields)[1] > 0
+def _two_step(kern: LBMKernel, d: _lib.LbmDesc, fields: bool) -> bool:
+    """Whether lbm_step runs this call's sequences (descriptor d: walls,
+    force, faces) two units per launch (the same choice, so the rollout's
+    forward launches match its own)."""
+    if kern.D != 3:
+        return False
+    nl, npair = ctypes.c_int(0), ctypes.c_int(0)
+    _lib.check(_lib.lib().lbm_step_schedule(ctypes.byref(d), 2, int(bool(fields)),
+                                            ctypes.byref(nl), ctypes.byref(npair)),
+               "lbm_step_schedule")
+    return npair.value > 0
 
 
 def _advance(L, st, d, kern, f0, wanted, nsteps, m, links, ff, tf, pairs, fN, moments):
@@ -148,7 +155,7 @@ def _forward_impl(ctx, f0, force, tau_field, rho_in, rho_out, kern: LBMKernel, n
     rho = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
     u = torch.empty(kern.grid_shape + (kern.D,), dtype=torch.float32, device=dev)
     p = torch.empty(kern.grid_shape, dtype=torch.float32, device=dev)
-    pairs = _two_step(kern, ff is not None or tf is not None)
+    pairs = _two_step(kern, d, ff is not None or tf is not None)
     tail = []
     for k, g in _advance(L, st, d, kern, f0, sorted(keep | tail_idx), nsteps, m, links, ff,
                          tf, pairs, fN, (rho, u, p)):

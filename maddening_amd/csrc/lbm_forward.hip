@@ -633,6 +633,11 @@ inline bool pair_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* p
   if (d->storage == LBM_F64) return false;
   if (!pair_applies(d, g, pa)) return false;
   if (d->flags & LBM_FLAG_PAIR_FORCE) return true;
+  // Zou-He faces: the faces form spills registers and lost to one step per
+  // launch (128x128x256 sphere channel with z faces, f32: 0.149 against
+  // 0.118 ms per step); a single update still takes it (0.20 against 0.37
+  // ms for push + faces + moments)
+  if (d->inlet_face > 0 || d->outlet_face > 0) return false;
   const bool wide = d->storage == LBM_F32 || d->storage == LBM_F32_DEV;
   if (!wide && (fields || d->bb_mode != LBM_BB_NONE || d->inlet_face > 0 ||
                 d->outlet_face > 0 || d->force[0] != 0.0 || d->force[1] != 0.0 ||
