@@ -837,15 +837,19 @@ def main():
 
     kern = make(args.storage)
     f0 = kern.equilibrium(rho, u)
+    # Dominant kernel first: the two-step launch, timed on its own (two cell
+    # updates per cell, the state read and written once). After an idle
+    # second the device needs some 20 ms of load to reach the clocks a long
+    # run holds (a 256^3 20-step call measured 0.306 ms/step right after a
+    # 5-step warm-up, 0.276 after 60 ms of launches, 0.314 after 1 s idle),
+    # so these 100 launches also bring the K-step call below to those clocks.
+    _, pairs0 = kern.schedule(args.steps)
+    per_launch_s = time_pair_launch(kern, f0, reps=100) if pairs0 else None
     ev, wall, launches, pairs, out = time_kernel(kern, f0, args.steps, args.warmup)
     mlups = cells * args.steps / ev / 1e6
     b_cell = 2 * 19 * STORAGE_BYTES[args.storage]
     rv = auto_row_vec(args.storage, n, args.row_vec) if kern.scheme != "push" else 0
-    if pairs:
-        # dominant kernel: the two-step launch, timed on its own; it performs
-        # two cell updates per cell and reads and writes the state once
-        per_launch_s = time_pair_launch(kern, f0)
-    else:
+    if not pairs:
         per_launch_s = ev / launches
     updates = 2 if pairs else 1
     # Algorithmic HBM bytes of one launch: the state read once and written
