@@ -117,12 +117,22 @@ struct PairAux {
 // raised by what only a later form needs.
 enum { PAIR_PLAIN = 0, PAIR_FORCE = 1, PAIR_GEN = 2, PAIR_GEN_FACES = 3 };
 
+// What a wave does in step 2. PAIR_MIXED: decided per lane. When whole rows
+// fill whole waves (w a multiple of 64), every lane of a wave is in the same
+// row, so the wave is either an inner row (PAIR_INNER: step 2 and its stores
+// on every plane) or a halo row or a thread past the rows (PAIR_HALO: step 1
+// only). A fixed role keeps the stores out of a branch: the memory counter
+// then counts a known number of stores, and the wait for the next plane's
+// gather at the end of an iteration no longer waits for the stores the
+// iteration just issued.
+enum { PAIR_MIXED = 0, PAIR_INNER = 1, PAIR_HALO = 2 };
+
 // The march of one workgroup over c-tile ct, b-tile bt and a-chunk `chunk`
 // of lattice a; xx is the workgroup's exchange buffer, two halves of 64 KB. With whole rows
 // (w == nc) the periodic wrap along c stays inside the workgroup; with
 // c-tiles the halo columns compute step 1 only, and step 2 of the tc inner
 // columns reads them.
-template <class T, int S1, int S2, int FORM>
+template <class T, int S1, int S2, int FORM, int ROLE = PAIR_MIXED>
 __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, int chunk,
                                            float (&xx)[2][16][1024]) {
   using L = Lattice<19>;
@@ -170,12 +180,13 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
   // column before the periodic wrap: -1 .. ctiles * tc, below 2 * nc as nc >= tc
   const int cl = ct * a.tc - a.halo + j;
   const int c = cl < 0 ? cl + nc : (cl >= nc ? cl - nc : cl);
-  const bool active = r < a.tb + 2;
+  const bool active = ROLE == PAIR_INNER || r < a.tb + 2;
   // a thread past the workgroup's rows takes the last row's cell: its
   // (unused) gathers then hit lines the workgroup reads anyway
   const int b = (b0 - 1 + (active ? r : a.tb + 1) + nb) % nb;
-  const bool inner = r >= 1 && r <= a.tb && b0 + r - 1 < nb && j >= a.halo &&
-                     j < a.halo + a.tc && cl < nc;
+  const bool inner = ROLE == PAIR_INNER ||
+                     (ROLE == PAIR_MIXED && r >= 1 && r <= a.tb && b0 + r - 1 < nb &&
+                      j >= a.halo && j < a.halo + a.tc && cl < nc);
   const T* src = static_cast<const T*>(a.src);
   T* dst = static_cast<T*>(a.dst);
   const int plane = nb * nc;
@@ -260,25 +271,60 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
 #pragma unroll
   for (int q = 0; q < 19; ++q) Z[q] = P1[q] = P2[q] = C(0);
   if (LEAN || active) fetch(a0 - 1, nx, nau);
-  for (int p = a0 - 1; p <= a1; ++p) {
-    C d[19];
+  // step 1 at plane p on the fetched values. LEAN: the next plane's gather
+  // is issued first, so it overlaps this plane's arithmetic; on the last
+  // plane it repeats plane a1, and every thread issues it (those past the
+  // workgroup's rows load the last row's cells again), so every plane
+  // issues the same loads.
+  auto step1 = [&](int p, C (&d)[19]) {
 #pragma unroll
     for (int q = 0; q < 19; ++q) d[q] = nx[q];
     if constexpr (GEN) cur = nau;
-    // LEAN: the next plane's gather, issued before this plane's arithmetic;
-    // on the last plane a repeat of plane a1, and by every thread (those
-    // past the workgroup's rows load the last row's cells again), so every
-    // plane issues the same loads
     if constexpr (LEAN) fetch(p < a1 ? p + 1 : a1, nx, nau);
-    {
-      if constexpr (S1 == PAIR_GATHER) walls_faces(d, cur, (p + na) % na);
-      C drho, rho, ua, ub, uc, Fa, Fb, Fc;
-      force_at((((p + na) % na) * nb + b) * nc + c, Fa, Fb, Fc);
-      moments<L, C, HASF, LS>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
-      collide_for<L, T, C, HASF, SCALED>(d, inv_tau_of(cur), Fa, Fb, Fc, drho, rho, ua, ub, uc);
-      as_stored<L, T, C>(d);  // the values a store and reload would give
+    if constexpr (S1 == PAIR_GATHER) walls_faces(d, cur, (p + na) % na);
+    C drho, rho, ua, ub, uc, Fa, Fb, Fc;
+    force_at((((p + na) % na) * nb + b) * nc + c, Fa, Fb, Fc);
+    moments<L, C, HASF, LS>(d, Fa, Fb, Fc, drho, rho, ua, ub, uc);
+    collide_for<L, T, C, HASF, SCALED>(d, inv_tau_of(cur), Fa, Fb, Fc, drho, rho, ua, ub, uc);
+    as_stored<L, T, C>(d);  // the values a store and reload would give
+  };
+  // the own column's values of this plane become those of the last one
+  auto carry = [&](const C (&d)[19]) {
+    if constexpr (GEN) prev = cur;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      if (L::ea[q] > 0) {
+        P2[q] = P1[q];
+        P1[q] = d[q];
+      }
+      if (L::ea[q] == 0) Z[q] = d[q];
     }
-    const bool out = p > a0;
+  };
+  // Planes a0 - 1 and a0 take step 1 only (step 2 on them would write
+  // planes outside the chunk). With a fixed role they run before the loop,
+  // so the loop body has no branch around step 2.
+  constexpr bool PEEL = ROLE != PAIR_MIXED;
+  if constexpr (PEEL) {
+#pragma unroll 1
+    for (int p = a0 - 1; p <= a0; ++p) {
+      C d[19];
+      step1(p, d);
+      if constexpr (!LEAN) {
+        if (p < a1 && active) fetch(p + 1, nx, nau);
+      }
+      carry(d);
+    }
+    // The loop's first step 1 consumes the gather in flight anyway. Waiting
+    // for it here means the loop header's memory waits are set by the loop's
+    // own order (gather, then stores) alone; merged with this path, whose
+    // newest operations are loads, they would wait for the stores too.
+    // s_waitcnt vmcnt(0), expcnt and lgkmcnt left at their maxima.
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+  }
+  for (int p = PEEL ? a0 + 1 : a0 - 1; p <= a1; ++p) {
+    C d[19];
+    step1(p, d);
+    const bool out = PEEL || p > a0;
     // planes alternate between the two halves of the exchange buffer, so the
     // one barrier per plane both publishes this plane's values and retires
     // the reads of the half written next
@@ -307,7 +353,7 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
       }
       walls_faces(e, prev, p - 1);
     };
-    if (out && inner) {
+    if (ROLE != PAIR_HALO && out && inner) {
       C e[19];
       gather2(e);
       const int cell = ((p - 1) * nb + b) * nc + c;
@@ -330,15 +376,10 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
 #pragma unroll
       for (int q = 0; q < 19; ++q) St::put(dst + q * P + cell, enc_u<L, T, SCALED>(q, e[q]));
     }
-    if constexpr (GEN) prev = cur;
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      if (L::ea[q] > 0) {
-        P2[q] = P1[q];
-        P1[q] = d[q];
-      }
-      if (L::ea[q] == 0) Z[q] = d[q];
-    }
+    // keeps the scheduler from mixing the next plane's step 1 into step 2,
+    // which needs more than the 128 registers a 1024-thread workgroup has
+    if constexpr (PEEL) __builtin_amdgcn_sched_barrier(0);
+    carry(d);
   }
 }
 
@@ -438,7 +479,21 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   __shared__ float x[2][16][1024];
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   const int rest = lb / a.ctiles;
-  pair_march<T, S1, S2, FORM>(a, lb - rest * a.ctiles, rest % a.btiles, rest / a.btiles, x);
+  const int ct = lb - rest * a.ctiles, bt = rest % a.btiles, chunk = rest / a.btiles;
+  if constexpr (FORM <= PAIR_FORCE && S2 == PAIR_COLLIDE) {
+    if (!a.halo && a.w % 64 == 0) {
+      // whole rows in whole waves: one role per wave (the barrier count per
+      // plane is the same in both roles)
+      const int r = (int)threadIdx.x / a.w;
+      const bool in = r >= 1 && r <= a.tb && bt * a.tb + r - 1 < a.nb;
+      if (__builtin_amdgcn_readfirstlane(in ? 1 : 0))
+        pair_march<T, S1, S2, FORM, PAIR_INNER>(a, ct, bt, chunk, x);
+      else
+        pair_march<T, S1, S2, FORM, PAIR_HALO>(a, ct, bt, chunk, x);
+      return;
+    }
+  }
+  pair_march<T, S1, S2, FORM>(a, ct, bt, chunk, x);
 }
 
 // The form a launch needs (PAIR_PLAIN .. PAIR_GEN_FACES).
