@@ -716,7 +716,12 @@ int launch_pair_gen(hipStream_t s, const PairArgs& pa) {
   const dim3 grid((unsigned)pa.nblocks), block(1024);
   switch (pair_form(pa)) {
     case PAIR_PLAIN:
-      hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_PLAIN>), grid, block, 0, s, pa);
+      // buffer stores when every population and a plane past the last end
+      // within 4 GiB of the array's base
+      if (20LL * pa.pitch * (long long)sizeof(T) < (1LL << 32))
+        hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_PLAIN, true>), grid, block, 0, s, pa);
+      else
+        hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_PLAIN>), grid, block, 0, s, pa);
       break;
     case PAIR_FORCE:
       hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_FORCE>), grid, block, 0, s, pa);

@@ -117,6 +117,30 @@ struct PairAux {
 // raised by what only a later form needs.
 enum { PAIR_PLAIN = 0, PAIR_FORCE = 1, PAIR_GEN = 2, PAIR_GEN_FACES = 3 };
 
+// Buffer stores for the plain march: one resource for the output array (its
+// base, a 4 GiB range, untyped dwords) and two 32-bit offsets, the cell's
+// place in a plane (per thread, fixed through the march) and the population
+// and plane (uniform), where a 64-bit address costs a vector add per store.
+// 256^3 per two-step launch: float16 380 -> 366 us, bfloat16 351 -> 331, the
+// 32-bit types unchanged (496-499 us). The same for the gather was slower
+// (f32dev 514 us, float16 478), so it keeps scalar plane bases with 32-bit
+// per-thread offsets. The launch uses it when the array's 19 populations and
+// a plane end within 4 GiB of its base.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t state_rsrc(const void* p) {
+  // word 3 of a gfx9 buffer resource for untyped dword access
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xFFFFFFFF,
+                                           0x00020000);
+}
+// non-temporal store (cache policy bit 1: slc / nt), as St::put
+template <class T>
+__device__ __forceinline__ void buf_store(T x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 2, "4- or 2-byte storage");
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, (int)v, (int)s, 2);
+  else
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, x), r, (int)v, (int)s, 2);
+}
+
 // What a wave does in step 2. PAIR_MIXED: decided per lane. When whole rows
 // fill whole waves (w a multiple of 64), every lane of a wave is in the same
 // row, so the wave is either an inner row (PAIR_INNER: step 2 and its stores
@@ -131,8 +155,8 @@ enum { PAIR_MIXED = 0, PAIR_INNER = 1, PAIR_HALO = 2 };
 // of lattice a; xx is the workgroup's exchange buffer, two halves of 64 KB. With whole rows
 // (w == nc) the periodic wrap along c stays inside the workgroup; with
 // c-tiles the halo columns compute step 1 only, and step 2 of the tc inner
-// columns reads them.
-template <class T, int S1, int S2, int FORM, int ROLE = PAIR_MIXED>
+// columns reads them. BUF: the plain form stores through a buffer resource.
+template <class T, int S1, int S2, int FORM, int ROLE = PAIR_MIXED, bool BUF = false>
 __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, int chunk,
                                            float (&xx)[2][16][1024]) {
   using L = Lattice<19>;
@@ -204,6 +228,7 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
 #pragma unroll
       for (int k = 0; k < 3; ++k) off[i][k] = (uint32_t)((Bs[i] * nc + Cs[k]) * (int)sizeof(T));
   }
+  const __amdgpu_buffer_rsrc_t rdst = state_rsrc(a.dst);
   // population q of plane ia (0 <= ia < na), as a uniform byte pointer
   auto qplane = [&](const void* base, int q, int ia) {
     return static_cast<const char*>(base) + ((size_t)q * P + (size_t)ia * plane) * sizeof(T);
@@ -373,8 +398,16 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
       } else {
         collide_for<L, T, C, HASF, SCALED>(e, inv_tau_of(prev), Fa, Fb, Fc, drho, rho, ua, ub, uc);
       }
+      if constexpr (LEAN && BUF) {
+        const uint32_t po = (uint32_t)((p - 1) * plane * (int)sizeof(T));
 #pragma unroll
-      for (int q = 0; q < 19; ++q) St::put(dst + q * P + cell, enc_u<L, T, SCALED>(q, e[q]));
+        for (int q = 0; q < 19; ++q)
+          buf_store<T>(enc_u<L, T, SCALED>(q, e[q]), rdst, off[1][1],
+                       (uint32_t)q * (uint32_t)P * (uint32_t)sizeof(T) + po);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) St::put(dst + q * P + cell, enc_u<L, T, SCALED>(q, e[q]));
+      }
     }
     // keeps the scheduler from mixing the next plane's step 1 into step 2,
     // which needs more than the 128 registers a 1024-thread workgroup has
@@ -474,7 +507,7 @@ __device__ __forceinline__ void pair_march_ring(const PairArgs& a, int bt, int c
   }
 }
 
-template <class T, int S1, int S2, int FORM>
+template <class T, int S1, int S2, int FORM, bool BUF = false>
 __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   __shared__ float x[2][16][1024];
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
@@ -487,13 +520,13 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
       const int r = (int)threadIdx.x / a.w;
       const bool in = r >= 1 && r <= a.tb && bt * a.tb + r - 1 < a.nb;
       if (__builtin_amdgcn_readfirstlane(in ? 1 : 0))
-        pair_march<T, S1, S2, FORM, PAIR_INNER>(a, ct, bt, chunk, x);
+        pair_march<T, S1, S2, FORM, PAIR_INNER, BUF>(a, ct, bt, chunk, x);
       else
-        pair_march<T, S1, S2, FORM, PAIR_HALO>(a, ct, bt, chunk, x);
+        pair_march<T, S1, S2, FORM, PAIR_HALO, BUF>(a, ct, bt, chunk, x);
       return;
     }
   }
-  pair_march<T, S1, S2, FORM>(a, ct, bt, chunk, x);
+  pair_march<T, S1, S2, FORM, PAIR_MIXED, BUF>(a, ct, bt, chunk, x);
 }
 
 // The form a launch needs (PAIR_PLAIN .. PAIR_GEN_FACES).
