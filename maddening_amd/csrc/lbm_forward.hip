@@ -580,9 +580,13 @@ inline bool pair_geometry(const lbm_desc* d, const Geom& g, PairArgs* pa) {
   pa->btiles = (int)btiles;
   pa->la = la;
   pa->nblocks = (int)blocks(la);
-  // runs of 16 workgroups per XCD (256^3, chunks of 64 planes: 655 us against
-  // 696 us in identity order; chunks of 128: 646 us, runs of 32: 648 us)
-  pa->remap = 16;
+  // runs of 16 workgroups per XCD for whole rows (256^3, chunks of 64
+  // planes: 655 us against 696 us in identity order; chunks of 128: 646 us,
+  // runs of 32: 648 us). C-tiles take runs of 32, two row bands of every
+  // tile column, so the halo rows between the two stay in one L2
+  // (128x128x1024: reads 1.52 -> 1.41 x the state, 583 -> 580 us; runs of
+  // 48-160 read 1.48-1.55 x and took 590-613 us)
+  pa->remap = tiled ? 32 : 16;
   if (d->flags & LBM_FLAG_XCD_REMAP) pa->remap = d->xcd_group > 0 ? d->xcd_group : pa->nblocks / 8;
   pa->inv_tau = (float)(1.0 / d->tau);
   pa->Fa = (float)d->force[0];
