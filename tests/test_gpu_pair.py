@@ -56,6 +56,32 @@ def test_pair_launches_equal_single_launches_bitwise(shape, storage, nsteps):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("shape", [(1024, 256, 256), (64, 1024, 1024)])
+def test_plain_form_above_4_gib_equals_single_launches_bitwise(shape):
+    """The plain form stores through a buffer resource when the output array
+    fits a 4 GiB range; 67,108,864 cells of float32 (5.1 GB per state) take
+    its other store path: whole rows with one role per wave, and c-tiles."""
+    from maddening_amd.lbm import LBMKernel
+    print(f"plain pair vs single above 4 GiB: {shape}", flush=True)
+    out = {}
+    for pair in ("force", "off"):
+        k = LBMKernel(shape, lattice="D3Q19", tau=0.7, storage="f32dev", scheme="pull", pair=pair)
+        rho = 1 + 0.01 * torch.randn(shape, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+        u = 0.03 * torch.randn(shape + (3,), device="cuda", generator=torch.Generator("cuda").manual_seed(4))
+        f = k.equilibrium(rho, u)
+        del rho, u
+        out[pair] = k.step(f, 5)
+        if pair == "force":
+            # (C, SC), (SC, SC), (SC, S)
+            assert k.schedule(5) == (3, 3)
+        del f, k
+    torch.cuda.synchronize()
+    for a, b in zip(out["force"], out["off"]):
+        assert torch.equal(a, b)
+    del out
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("shape", [(64, 80, 128), (33, 50, 64), (2, 2, 3), (5, 14, 512)])
 @pytest.mark.parametrize("storage", ["f16dev", "bf16dev"])
 @pytest.mark.parametrize("nsteps", [2, 5])
