@@ -731,7 +731,10 @@ int launch_pair_gen(hipStream_t s, const PairArgs& pa) {
       hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_FORCE>), grid, block, 0, s, pa);
       break;
     case PAIR_GEN:
-      hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_GEN>), grid, block, 0, s, pa);
+      if (20LL * pa.pitch * (long long)sizeof(T) < (1LL << 32))
+        hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_GEN, true>), grid, block, 0, s, pa);
+      else
+        hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_GEN>), grid, block, 0, s, pa);
       break;
     default:
       hipLaunchKernelGGL((k_pair<T, S1, S2, PAIR_GEN_FACES>), grid, block, 0, s, pa);

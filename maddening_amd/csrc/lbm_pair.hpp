@@ -117,15 +117,18 @@ struct PairAux {
 // raised by what only a later form needs.
 enum { PAIR_PLAIN = 0, PAIR_FORCE = 1, PAIR_GEN = 2, PAIR_GEN_FACES = 3 };
 
-// Buffer stores for the plain march: one resource for the output array (its
-// base, a 4 GiB range, untyped dwords) and two 32-bit offsets, the cell's
-// place in a plane (per thread, fixed through the march) and the population
-// and plane (uniform), where a 64-bit address costs a vector add per store.
-// 256^3 per two-step launch: float16 380 -> 366 us, bfloat16 351 -> 331, the
-// 32-bit types unchanged (496-499 us). The same for the gather was slower
-// (f32dev 514 us, float16 478), so it keeps scalar plane bases with 32-bit
-// per-thread offsets. The launch uses it when the array's 19 populations and
-// a plane end within 4 GiB of its base.
+// Buffer stores for the plain and general marches: one resource for the
+// output array (its base, a 4 GiB range, untyped dwords) and two 32-bit
+// offsets, the cell's place in a plane (per thread, fixed through the march)
+// and the population and plane (uniform), where a 64-bit address costs a
+// vector add per store. 256^3 per two-step launch: float16 380 -> 366 us,
+// bfloat16 351 -> 331, the 32-bit types unchanged (496-499 us); with fixed
+// wave roles, the config-4 sphere channel (general form) 0.0841-0.0846 ->
+// 0.0818-0.0824 ms per step. The uniform-force form spills with them (573 ->
+// 605 us at 256^3) and keeps 64-bit addresses. The same for the gather was
+// slower (f32dev 514 us, float16 478), so it keeps scalar plane bases with
+// 32-bit per-thread offsets. The launch uses it when the array's 19
+// populations and a plane end within 4 GiB of its base.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t state_rsrc(const void* p) {
   // word 3 of a gfx9 buffer resource for untyped dword access
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xFFFFFFFF,
@@ -155,7 +158,8 @@ enum { PAIR_MIXED = 0, PAIR_INNER = 1, PAIR_HALO = 2 };
 // of lattice a; xx is the workgroup's exchange buffer, two halves of 64 KB. With whole rows
 // (w == nc) the periodic wrap along c stays inside the workgroup; with
 // c-tiles the halo columns compute step 1 only, and step 2 of the tc inner
-// columns reads them. BUF: the plain form stores through a buffer resource.
+// columns reads them. BUF: stores through a buffer resource (the plain and
+// general forms).
 template <class T, int S1, int S2, int FORM, int ROLE = PAIR_MIXED, bool BUF = false>
 __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, int chunk,
                                            float (&xx)[2][16][1024]) {
@@ -398,11 +402,12 @@ __device__ __forceinline__ void pair_march(const PairArgs& a, int ct, int bt, in
       } else {
         collide_for<L, T, C, HASF, SCALED>(e, inv_tau_of(prev), Fa, Fb, Fc, drho, rho, ua, ub, uc);
       }
-      if constexpr (LEAN && BUF) {
+      if constexpr (BUF) {
         const uint32_t po = (uint32_t)((p - 1) * plane * (int)sizeof(T));
+        const uint32_t co = LEAN ? off[1][1] : (uint32_t)((b * nc + c) * (int)sizeof(T));
 #pragma unroll
         for (int q = 0; q < 19; ++q)
-          buf_store<T>(enc_u<L, T, SCALED>(q, e[q]), rdst, off[1][1],
+          buf_store<T>(enc_u<L, T, SCALED>(q, e[q]), rdst, co,
                        (uint32_t)q * (uint32_t)P * (uint32_t)sizeof(T) + po);
       } else {
 #pragma unroll
@@ -513,7 +518,7 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   const int lb = xcd_remap(blockIdx.x, a.nblocks, a.remap);
   const int rest = lb / a.ctiles;
   const int ct = lb - rest * a.ctiles, bt = rest % a.btiles, chunk = rest / a.btiles;
-  if constexpr (FORM <= PAIR_FORCE && S2 == PAIR_COLLIDE) {
+  if constexpr (FORM <= PAIR_GEN && S2 == PAIR_COLLIDE) {
     if (!a.halo && a.w % 64 == 0) {
       // whole rows in whole waves: one role per wave (the barrier count per
       // plane is the same in both roles)
