@@ -80,7 +80,7 @@ def load_kernels(kern, steps):
     k = LBMKernel(shape, lattice=kern.lat.name, tau=kern.tau, storage=kern.storage,
                   scheme=kern.scheme, block_x=kern.block_x, row_vec=kern.row_vec,
                   xcd_remap=kern.xcd_remap, pair="force" if kern.pair != "off" else "off",
-                  device=kern.device)
+                  tri=kern.tri, device=kern.device)
     k.step(k.equilibrium(), steps)
     torch.cuda.synchronize()
 
@@ -878,7 +878,11 @@ def main():
         "config": {"workload": f"D3Q19 BGK periodic {n}^3 Taylor-Green vortex, u0=0.05, "
                                f"tau=0.8, populations stored as {args.storage}",
                    "cells": cells, "scheme": kern.scheme, "launches": launches,
-                   "two_step_launches": pairs, "compute_dtype": "f64" if args.storage == "f64"
+                   "two_step_launches": pairs,
+                   # the first launch runs three units (an even K, LBM_FLAG_TRI_OFF)
+                   "three_unit_launches": int(bool(pairs) and args.steps % 2 == 0 and
+                                              args.steps >= 4 and launches == args.steps // 2),
+                   "compute_dtype": "f64" if args.storage == "f64"
                    else "f32", "wall_s_timed_call": round(wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

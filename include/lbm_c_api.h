@@ -130,6 +130,15 @@ typedef struct lbm_desc {
  * are bit-identical either way. */
 #define LBM_FLAG_PAIR_OFF 16
 #define LBM_FLAG_PAIR_FORCE 32
+/* Three units in the first launch (no counterpart in the reference): a
+ * D3Q19 sequence run two units per launch has an odd number of units when
+ * nsteps is even (4 or more), and lbm_step then runs the collision and the
+ * first two fused steps as one three-unit launch instead of a two-unit
+ * launch and a single one, where it applies: LBM_F32_DEV, LBM_BB_NONE, no
+ * force and no fields, dims[2] a multiple of 64, the state within 4 GiB
+ * (19 * dims product * 4 bytes and a plane more). Bit-identical; TRI_OFF
+ * turns it off. */
+#define LBM_FLAG_TRI_OFF 64
 
 /* ---- queries ----------------------------------------------------------- */
 
@@ -178,7 +187,13 @@ int lbm_step(void* stream, const lbm_desc* d, const void* f_in, void* f_out,
  *            into rho/u/p (each nullable);
  *   phase 3: src = post-collision g_{n-1}, dst = C(f_{n+1}): two phase-1
  *            launches in one, bit for bit (LBM_FLAG_PAIR_*); returns
- *            LBM_ERR_UNSUPPORTED where the two-step launch does not apply.
+ *            LBM_ERR_UNSUPPORTED where the two-step launch does not apply;
+ *   phases 4, 5, 6: the other D3Q19 two-unit launches, as phase 0 then 1,
+ *            phase 1 then 2 (rho/u/p as in phase 2), and phase 0 then 2;
+ *   phase 7: src = a state f_n, dst = C(f_{n+2}): phase 0 and two phase-1
+ *            launches in one, bit for bit, where LBM_FLAG_TRI_OFF's rule
+ *            allows it but for the row length (any dims[2] >= 64);
+ *            LBM_ERR_UNSUPPORTED elsewhere.
  * links (nullable, LBM_BB_HALFWAY only): the link words lbm_links_build made
  * from this mask; the gathers then read one word per cell instead of the
  * neighbours' mask bytes (the same results).
@@ -189,9 +204,9 @@ int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, 
 
 /*
  * Launches lbm_step makes for nsteps steps with this descriptor (with_fields:
- * the call passes a force or tau field), and how many of them are two-step
- * launches (LBM_FLAG_PAIR_*). Pull sequence: collide, two-step launches,
- * single fused launches, stream + moments. Push scheme: one launch per step
+ * the call passes a force or tau field), and how many of them run two or
+ * three units (LBM_FLAG_PAIR_*, LBM_FLAG_TRI_OFF). Pull sequence: collide,
+ * two-step launches, single fused launches, stream + moments. Push scheme: one launch per step
  * plus the moments pass (Zou-He faces add one launch per face and step).
  * No counterpart in the reference; for timing and profiling.
  */

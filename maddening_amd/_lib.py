@@ -23,6 +23,7 @@ LBM_FLAG_ADJ_OUT_SWAPPED = 4
 LBM_FLAG_ADJ_OUT_FACES = 8
 LBM_FLAG_PAIR_OFF = 16
 LBM_FLAG_PAIR_FORCE = 32
+LBM_FLAG_TRI_OFF = 64
 
 STORAGE_NAMES = {"f32": LBM_F32, "f64": LBM_F64, "f16dev": LBM_F16_DEV,
                  "bf16dev": LBM_BF16_DEV, "f32dev": LBM_F32_DEV}

@@ -28,6 +28,7 @@
 #include "lbm_host.hpp"
 #include "lbm_cell.hpp"
 #include "lbm_pair.hpp"
+#include "lbm_tri.hpp"
 
 namespace mlbm {
 
@@ -595,6 +596,72 @@ inline bool pair_geometry(const lbm_desc* d, const Geom& g, PairArgs* pa) {
   return true;
 }
 
+// The three-step march (k_tri) for float32 deviation storage, periodic,
+// without force or fields: c-tiles of 64 output columns with two halo
+// columns each side (rows of at least 64 cells), the output array within
+// the 4 GiB range of its buffer stores. The chunk length minimises rounds of
+// one-workgroup-per-CU launches x (chunk + 4) planes.
+inline bool tri_geometry(const lbm_desc* d, const Geom& g, PairArgs* pa) {
+  if (d->q != 19 || d->storage != LBM_F32_DEV || d->bb_mode != LBM_BB_NONE) return false;
+  if (d->force[0] != 0.0 || d->force[1] != 0.0 || d->force[2] != 0.0) return false;
+  if (g.nc < kTriTile) return false;
+  if (19LL * g.plane >= (1LL << 31)) return false;
+  if (20LL * g.plane * 4 >= (1LL << 32)) return false;
+  const long long ctiles = (g.nc + kTriTile - 1) / kTriTile;
+  const long long btiles = (g.nb + kTriOut - 1) / kTriOut;
+  auto blocks = [&](int l) { return ctiles * btiles * ((g.na + l - 1) / l); };
+  int la = 128;
+  long long best = -1;
+  for (int l = 128; l >= 8; l /= 2) {
+    const long long cost = ((blocks(l) + kPairCUs - 1) / kPairCUs) * (l + 4);
+    if (best < 0 || cost < best) {
+      best = cost;
+      la = l;
+    }
+  }
+  if (blocks(la) >= (1LL << 31)) return false;
+  std::memset(pa, 0, sizeof(*pa));
+  pa->na = g.na;
+  pa->nb = g.nb;
+  pa->nc = g.nc;
+  pa->pitch = (int)g.plane;
+  pa->w = kTriW;
+  pa->tc = kTriTile;
+  pa->halo = 2;
+  pa->ctiles = (int)ctiles;
+  pa->tb = kTriOut;
+  pa->btiles = (int)btiles;
+  pa->la = la;
+  pa->nblocks = (int)blocks(la);
+  pa->remap = 16;
+  if (d->flags & LBM_FLAG_XCD_REMAP) pa->remap = d->xcd_group > 0 ? d->xcd_group : pa->nblocks / 8;
+  pa->inv_tau = (float)(1.0 / d->tau);
+  return true;
+}
+
+// Where lbm_step folds the collision and the first two fused steps into one
+// launch (an even nsteps >= 4 of a sequence run two units per launch): the
+// geometry above with whole 64-column tiles, unless LBM_FLAG_TRI_OFF. 256^3
+// f32dev: 760 us for the three units against 485 + 450 us for (C, SC) and
+// a single SC launch; in the middle of a sequence (SC, SC, SC) was no
+// faster per unit than (SC, SC) (253 against 248-253 us), so it is used
+// only there.
+inline bool tri_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
+  if (fields || (d->flags & LBM_FLAG_TRI_OFF) || g.nc % kTriTile != 0) return false;
+  return tri_geometry(d, g, pa);
+}
+
+template <class T>
+int launch_tri_first(hipStream_t s, const PairArgs& pa) {
+  if constexpr (std::is_same<T, f32dev_t>::value) {
+    hipLaunchKernelGGL((k_tri<T, PAIR_LOCAL, PAIR_COLLIDE>), dim3((unsigned)pa.nblocks),
+                       dim3(1024), 0, s, pa);
+    return check_launch("k_tri launch");
+  } else {
+    return fail(LBM_ERR_UNSUPPORTED, "three-unit launches are for LBM_F32_DEV");
+  }
+}
+
 // Whether the two-step march can run this call at all: D3Q19, the reference
 // wall rule or none (not halfway bounce-back), and a lattice the geometry
 // covers.
@@ -786,16 +853,27 @@ void pair_general(PairArgs* pa, const FwdArgs<C>& a, bool bb_ref) {
 //    two-step launch. nsteps = 1: (C, S) in one launch; nsteps = 2: C, then
 //    (SC, S); nsteps >= 3: (C, SC), the odd SC when nsteps is even, pairs of
 //    SC, (SC, S).
+//  * With `tri` (an even nsteps >= 4 where tri_plan applies): (C, SC, SC),
+//    pairs of SC, (SC, S), every launch a multi-unit one.
 struct PullSchedule {
   int launches;   // all launches
-  int pairs;      // two-step launches among them
+  int pairs;      // launches of two or three units among them
   int npair;      // (SC, SC) launches
   int nsingle;    // single SC launches
   bool fold;      // the first and last units run inside two-step launches
+  bool tri;       // the first launch is (C, SC, SC)
 };
-inline PullSchedule pull_schedule(int nsteps, bool pair, bool fold) {
+inline PullSchedule pull_schedule(int nsteps, bool pair, bool fold, bool tri = false) {
   PullSchedule sc;
   sc.fold = pair && fold;
+  sc.tri = false;
+  if (sc.fold && tri && nsteps >= 4 && nsteps % 2 == 0) {
+    sc.tri = true;
+    sc.nsingle = 0;
+    sc.npair = (nsteps - 4) / 2;
+    sc.launches = sc.pairs = 2 + sc.npair;
+    return sc;
+  }
   if (!sc.fold) {
     const int fused = nsteps - 1;
     sc.npair = pair ? fused / 2 : 0;
@@ -829,11 +907,11 @@ template <class L, class T, int BB>
 int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
               const void* f_in, void* f_out, void* work, int nsteps,
               float* rho, float* u, float* p,
-              const typename PairOf<L>::Args* pair = nullptr) {
+              const typename PairOf<L>::Args* pair = nullptr, const PairArgs* tri = nullptr) {
   using C = typename Storage<T>::C;
   int rc;
   constexpr bool kFold = L::Q == 19 && (BB == BB_NONE || BB == BB_REF);
-  const PullSchedule sc = pull_schedule(nsteps, pair != nullptr, kFold);
+  const PullSchedule sc = pull_schedule(nsteps, pair != nullptr, kFold, tri != nullptr);
   void* buf[2] = {f_out, work};
   int j = 0;
   auto next = [&]() { return buf[(sc.launches - 1 - j++) & 1]; };
@@ -857,6 +935,19 @@ int step_pull(hipStream_t s, const Geom& g, FwdArgs<typename Storage<T>::C> a,
           k.dst = next();
           if ((rc = launch_cell<L, T, BB, false, true, true>(s, g, k))) return rc;
           cur = k.dst;
+        } else if (sc.tri) {
+          PairArgs ta = *tri;
+          ta.src = f_in;
+          ta.dst = next();
+          if ((rc = launch_tri_first<T>(s, ta))) return rc;
+          cur = ta.dst;
+          for (int i = 0; i < sc.npair; ++i) {
+            pa.src = cur;
+            pa.dst = next();
+            rc = launch_pair_gen<T, PAIR_GATHER, PAIR_COLLIDE>(s, pa);
+            if (rc) return rc;
+            cur = pa.dst;
+          }
         } else {
           pa.src = f_in;
           pa.dst = next();
@@ -966,17 +1057,19 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
     // two-step launches: a single update as one launch (pair_one), longer
     // sequences where pair_plan measured them faster (or PAIR_FORCE)
     typename PairOf<L>::Args pa;
-    bool pair = false;
+    PairArgs ta;
+    bool pair = false, tri = false;
     if constexpr (L::Q == 19) {
       if (nsteps == 1) pair = one_storage<L, T>() && pair_one(d, g, &pa);
       else pair = pair_storage<L, T>() && pair_plan(d, g, ff || tf, &pa);
+      tri = pair && nsteps >= 4 && std::is_same<T, f32dev_t>::value && tri_plan(d, g, ff || tf, &ta);
     } else {
       pair = pair_storage<L, T>() && PairOf<L>::plan(d, g, ff || tf, &pa);
     }
     switch (d->bb_mode) {
       case LBM_BB_NONE:
         return step_pull<L, T, BB_NONE>(s, gr, a, f_in, f_out, work, nsteps, rho, u, p,
-                                        pair ? &pa : nullptr);
+                                        pair ? &pa : nullptr, tri ? &ta : nullptr);
       case LBM_BB_REFERENCE:
         return step_pull<L, T, BB_REF>(s, g, a, f_in, f_out, work, nsteps, rho, u, p,
                                        pair ? &pa : nullptr);
@@ -1060,6 +1153,17 @@ struct PullPhaseOp {
         pa.dst = dst;
         return PairOf<L>::template launch<T>(s, pa);
       }
+    }
+    if (phase == 7) {
+      // D3Q19 three-unit launch (C, SC, SC)
+      PairArgs ta;
+      if (L::Q == 19 && !ff && !tf && tri_geometry(d, g, &ta)) {
+        ta.src = src;
+        ta.dst = dst;
+        return launch_tri_first<T>(s, ta);
+      }
+      return fail(LBM_ERR_UNSUPPORTED,
+                  "phase 7 (three units per launch) does not apply to this call");
     }
     if (phase >= 3) {
       // D3Q19 two-step launches: 3 (SC, SC), 4 (C, SC), 5 (SC, S), 6 (C, S)
@@ -1228,11 +1332,14 @@ static CallPlan call_plan(const lbm_desc* d, const Geom& g, int nsteps, bool fie
   if (cp.scheme == LBM_SCHEME_AUTO)
     cp.scheme = (nsteps <= 1 && !one) ? LBM_SCHEME_PUSH : LBM_SCHEME_PULL;
   cp.pair = false;
+  bool tri = false;
   if (cp.scheme == LBM_SCHEME_PULL) {
     if (d->q == 19) cp.pair = nsteps == 1 ? one : pair_plan(d, g, fields, &pa);
     else cp.pair = pair9_plan(d, g, fields, &pa9);
+    PairArgs ta;
+    tri = d->q == 19 && cp.pair && nsteps >= 4 && tri_plan(d, g, fields, &ta);
   }
-  cp.sc = pull_schedule(nsteps, cp.pair, d->q == 19 && d->bb_mode != LBM_BB_HALFWAY);
+  cp.sc = pull_schedule(nsteps, cp.pair, d->q == 19 && d->bb_mode != LBM_BB_HALFWAY, tri);
   return cp;
 }
 
@@ -1297,7 +1404,7 @@ int lbm_pull_phase(void* stream, const lbm_desc* d, int phase, const void* src, 
   Geom g;
   int rc = check_desc(d, &g);
   if (rc) return rc;
-  if (phase < 0 || phase > 6) return fail(LBM_ERR_INVALID, "phase must be 0 to 6");
+  if (phase < 0 || phase > 7) return fail(LBM_ERR_INVALID, "phase must be 0 to 7");
   if (!src || !dst) return fail(LBM_ERR_INVALID, "src and dst must be non-NULL");
   if (src == dst) return fail(LBM_ERR_INVALID, "src and dst must be distinct buffers");
   if (d->bb_mode != LBM_BB_NONE && !mask)

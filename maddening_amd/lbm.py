@@ -59,6 +59,8 @@ class LBMKernel:
     pair : "auto", "off" or "force": two pull steps per launch
         (LBM_FLAG_PAIR_*); "auto" uses them where they measured faster.
         Results are bit-identical in every mode.
+    tri : False sets LBM_FLAG_TRI_OFF (no three-unit first launch in even
+        sequences of float32 deviation storage); the same bits either way.
     """
 
     def __init__(self, grid_shape: Sequence[int], lattice: str = "D3Q19",
@@ -67,7 +69,7 @@ class LBMKernel:
                  lid_velocity: Sequence[float] = (0.0, 0.0, 0.0),
                  scheme: str = "auto", block_x: int = 0, row_vec: int = 0,
                  xcd_remap: bool = False, xcd_group: int = 0, pair: str = "auto",
-                 device="cuda"):
+                 tri: bool = True, device="cuda"):
         if lattice.upper() == "D3Q19":
             self.lat: LatticeDescriptor = d3q19()
         elif lattice.upper() == "D2Q9":
@@ -103,6 +105,7 @@ class LBMKernel:
         self.block_x = int(block_x)
         self.xcd_remap = bool(xcd_remap)
         self.xcd_group = int(xcd_group)
+        self.tri = bool(tri)
         self.row_vec = int(row_vec)
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
@@ -136,7 +139,8 @@ class LBMKernel:
             d.lid_velocity[i] = self.lid_velocity[i]
         d.block_x = self.block_x
         d.row_vec = self.row_vec
-        d.flags = (_lib.LBM_FLAG_XCD_REMAP if self.xcd_remap else 0) | _PAIR_FLAGS[self.pair]
+        d.flags = ((_lib.LBM_FLAG_XCD_REMAP if self.xcd_remap else 0) | _PAIR_FLAGS[self.pair]
+                   | (0 if self.tri else _lib.LBM_FLAG_TRI_OFF))
         d.xcd_group = self.xcd_group
         if faces is not None:
             for slot, spec in zip(("inlet", "outlet"), faces):
@@ -156,11 +160,15 @@ class LBMKernel:
     def empty(self) -> torch.Tensor:
         return torch.empty(self.state_shape, dtype=self.dtype, device=self.device)
 
-    def schedule(self, nsteps: int, with_fields: bool = False) -> tuple[int, int]:
-        """(launches, two-step launches) of ``step(f, nsteps)``; with_fields:
-        the call passes a force or tau field."""
+    def schedule(self, nsteps: int, with_fields: bool = False, force=None) -> tuple[int, int]:
+        """(launches, launches of two or three units) of ``step(f, nsteps)``;
+        with_fields: the call passes a force or tau field; force: the call's
+        uniform force (a force rules out the three-unit launch)."""
         nl, npair = ctypes.c_int(0), ctypes.c_int(0)
-        _lib.check(_lib.lib().lbm_step_schedule(ctypes.byref(self.desc()), int(nsteps),
+        if isinstance(force, torch.Tensor):
+            force = None if force.dim() > 1 else force.tolist()
+        d = self.desc(force=force)
+        _lib.check(_lib.lib().lbm_step_schedule(ctypes.byref(d), int(nsteps),
                                                 int(bool(with_fields)), ctypes.byref(nl),
                                                 ctypes.byref(npair)), "lbm_step_schedule")
         return nl.value, npair.value

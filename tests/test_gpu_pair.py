@@ -48,7 +48,7 @@ def _run(shape, storage, pair, nsteps, seed=5, force=(1e-5, -2e-5, 3e-5)):
 def test_pair_launches_equal_single_launches_bitwise(shape, storage, nsteps):
     print(f"pair vs single: {shape} {storage} nsteps={nsteps}", flush=True)
     k, forced = _run(shape, storage, "force", nsteps)
-    launches, pairs = k.schedule(nsteps)
+    launches, pairs = k.schedule(nsteps, force=[1e-5, -2e-5, 3e-5])
     # n + 1 units (collide, n - 1 fused steps, stream + moments), two per launch
     assert pairs == (nsteps + 1) // 2 and launches == (nsteps + 2) // 2
     _, single = _run(shape, storage, "off", nsteps)
@@ -290,4 +290,50 @@ def test_d2q9_auto_pairs_at_2048_squared_equal_single_launches():
         outs.append([t.cpu() for t in k.step(f, 9, force=[1e-6, 0.0])])
     torch.cuda.synchronize()
     for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
+# rows of whole 64-column tiles and ragged ones, b-tiles of 11 rows cut by
+# the lattice, chunks of planes cut by it, axes of one cell
+TRI_SHAPES = [(9, 20, 128), (7, 9, 100), (40, 37, 256), (5, 14, 512), (3, 5, 700), (1, 1, 64),
+              (1, 1, 68), (2, 3, 205), (12, 7, 204), (6, 30, 256), (2, 2, 70), (1, 12, 130),
+              (130, 12, 68)]
+
+
+def _phase(k, phase, src, dst):
+    from maddening_amd import _lib
+    from maddening_amd.lbm import _ptr, _stream
+    d = k.desc()
+    _lib.check(_lib.lib().lbm_pull_phase(_stream(k.device), ctypes.byref(d), phase, _ptr(src),
+                                         _ptr(dst), None, None, None, None, None, None, None),
+               f"lbm_pull_phase {phase}")
+
+
+@pytest.mark.parametrize("shape", TRI_SHAPES)
+def test_three_unit_launch_equals_three_single_launches_bitwise(shape):
+    """Phase 7, (C, SC, SC) in one launch, against phase 0 then phase 1 twice."""
+    from maddening_amd.lbm import LBMKernel
+    print(f"three units vs singles: {shape}", flush=True)
+    k = LBMKernel(shape, lattice="D3Q19", tau=0.7, storage="f32dev", scheme="pull")
+    f0 = _state(k, shape, 11)
+    a, b = k.empty(), k.empty()
+    _phase(k, 0, f0, a)
+    _phase(k, 1, a, b)
+    _phase(k, 1, b, a)
+    t = k.empty()
+    _phase(k, 7, f0, t)
+    torch.cuda.synchronize()
+    assert torch.equal(a, t)
+
+
+@pytest.mark.parametrize("shape,pair", [((3, 5, 64), "force"), ((4, 23, 192), "force"),
+                                        ((64, 128, 256), "auto")])
+@pytest.mark.parametrize("nsteps", [4, 6, 20])
+def test_even_sequences_with_a_three_unit_launch_equal_single_launches(shape, pair, nsteps):
+    print(f"lbm_step with (C, SC, SC): {shape} {pair} nsteps={nsteps}", flush=True)
+    k, multi = _run(shape, "f32dev", pair, nsteps, force=(0.0, 0.0, 0.0))
+    # (C, SC, SC), (nsteps - 4) / 2 pairs, (SC, S)
+    assert k.schedule(nsteps) == (nsteps // 2, nsteps // 2)
+    _, single = _run(shape, "f32dev", "off", nsteps, force=(0.0, 0.0, 0.0))
+    for a, b in zip(multi, single):
         assert torch.equal(a, b)
