@@ -642,7 +642,7 @@ inline bool tri_geometry(const lbm_desc* d, const Geom& g, PairArgs* pa) {
 // Where lbm_step folds the collision and the first two fused steps into one
 // launch (an even nsteps >= 4 of a sequence run two units per launch): the
 // geometry above with whole 64-column tiles, unless LBM_FLAG_TRI_OFF. 256^3
-// f32dev: 760 us for the three units against 485 + 450 us for (C, SC) and
+// f32dev: 699 us for the three units against 472 + 450 us for (C, SC) and
 // a single SC launch; in the middle of a sequence (SC, SC, SC) was no
 // faster per unit than (SC, SC) (253 against 248-253 us), so it is used
 // only there.

@@ -4,10 +4,15 @@
 //
 // The two-step march moves the state across HBM once per two updates and,
 // in 32-bit storage, is bound by that traffic: with its collisions taken out
-// it runs within 3 % of its time. k_tri computes g_{n+3} from g_n in one
-// pass, so the state crosses HBM once per three updates, and pays for it in
-// arithmetic: 3.67 collisions per output cell against the two-step march's
-// 3.0 at 256^3.
+// it runs within 3 % of its time. k_tri runs three units in one pass, so the
+// state crosses HBM once per three updates, and pays for it in arithmetic:
+// 3.67 collisions per output cell against the two-step march's 3.0 at 256^3.
+// In the middle of a sequence that came out even per unit (256^3 f32dev:
+// 253 us against 248-253 for the two-step launch; with the traffic gone the
+// march is bound by issue, two barriers per plane), so lbm_step uses it
+// only where it replaces a single-unit launch: the first launch of an even
+// sequence, collide plus two fused steps (S1 = PAIR_LOCAL), 699 us against
+// 472 + 450 us for a two-step launch and a single one.
 //
 // A 1024-thread workgroup owns 15 rows of a c-tile: 64 output columns plus
 // two halo columns each side (kTriW = 68 cells), and marches along axis a
