@@ -248,8 +248,10 @@ def precision_leg(n, dev, checkpoints=(100, 1000, 5000)):
 def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=16, reps=3):
     """Sphere-packed channel (porosity 0.7, radii 6-12, seed 0), tau field 0.7,
     body force (0, 0, 1e-5), loss = mean u_z over the outlet plane. Times the
-    float32 forward and the sqrt(N)-checkpointed adjoint; checks the float64
-    adjoint against central differences of the float64 forward."""
+    float32 forward and the sqrt(N)-checkpointed adjoint, and (f32) the
+    adjoint with every state kept, which is what the rollout's default
+    ("auto") does when the states fit in free device memory, as here; checks
+    the float64 adjoint against central differences of the float64 forward."""
     import torch
     from maddening_amd import workloads
     from maddening_amd.adjoint import rollout
@@ -280,7 +282,8 @@ def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=16, reps=3):
         k.step(f0, 10, mask=mask, force=[0, 0, F0], tau_field=tau, work=work)
         Fw = torch.tensor([0.0, 0.0, F0], dtype=torch.float64, device=dev, requires_grad=True)
         tw = tau.clone().requires_grad_(True)
-        fw, _, _, _ = rollout(k, f0, nsteps, force=Fw, tau_field=tw, mask=mask)
+        fw, _, _, _ = rollout(k, f0, nsteps, force=Fw, tau_field=tw, mask=mask,
+                              checkpoint="sqrt")
         uz_loss(k, fw, Fw[2]).backward()
         del fw, Fw, tw
         torch.cuda.synchronize()
@@ -297,7 +300,8 @@ def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=16, reps=3):
             tg = tau.clone().requires_grad_(True)
             torch.cuda.synchronize()
             e0.record()
-            fN, _, _, _ = rollout(k, f0, nsteps, force=Fg, tau_field=tg, mask=mask)
+            fN, _, _, _ = rollout(k, f0, nsteps, force=Fg, tau_field=tg, mask=mask,
+                                  checkpoint="sqrt")
             e1.record()
             L = uz_loss(k, fN, Fg[2])
             L.backward()
@@ -307,7 +311,8 @@ def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=16, reps=3):
             adj_t.append(e1.elapsed_time(e2) / nsteps)
             tot_t.append(e0.elapsed_time(e2) / nsteps)
         fwd_ms, adj_ms = statistics.median(fwd_t), statistics.median(adj_t)
-        rec = {"forward_ms_per_step": round(fwd_ms, 4),
+        rec = {"checkpoint": "sqrt",
+               "forward_ms_per_step": round(fwd_ms, 4),
                "rollout_forward_ms_per_step": round(statistics.median(roll_t), 4),
                "adjoint_ms_per_step": round(adj_ms, 4),
                "adjoint_over_forward": round(adj_ms / fwd_ms, 3),
@@ -320,6 +325,10 @@ def adjoint_leg(dev, nsteps=500, shape=(128, 128, 256), fd_cells=16, reps=3):
             # every post-collision state kept (N x 319 MB): no recompute
             del fN
             torch.cuda.empty_cache()
+            from maddening_amd.adjoint import resolve_checkpoint
+            # what the rollout's default ("auto") picks for this call here
+            rec["default_checkpoint"] = resolve_checkpoint(
+                k.ncells * 19 * 4, nsteps, "auto", device=dev)
             # one untimed all-states rollout first, so the timed one takes its
             # state buffers from torch's cache instead of fresh allocations
             for timed in (False, True):

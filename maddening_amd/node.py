@@ -257,7 +257,8 @@ class LBMNode(SimulationNode):
         Differentiable: when ``state["f"]``, a tensor ``body_force`` or a
         tensor ``inlet_pressure``/``outlet_pressure`` requires grad (and grad
         mode is on), the step runs as an autograd rollout whose backward is
-        the HIP adjoint (sqrt(N) checkpointing). That is the counterpart of
+        the HIP adjoint (every state kept when they fit in free device
+        memory, sqrt(N) checkpointing otherwise). That is the counterpart of
         jax.grad through the reference's update."""
         mask, force, faces = self._inputs(state, boundary_inputs)
         k, kmask = self._kernel_for(mask)

@@ -8,7 +8,8 @@ This is the PyTorch counterpart of the jax.ffi target the reference's
   tracing see the op without running it;
 * an autograd formula. It returns the gradients of f_0, of the force
   (uniform (D,) or per cell (*G, D)) and of the per-cell tau field through
-  the HIP adjoint (``maddening_amd.adjoint``, sqrt(N) checkpointing). When
+  the HIP adjoint (``maddening_amd.adjoint``: every state kept when they
+  fit in free device memory, sqrt(N) checkpointing otherwise). When
   an input requires grad, the op's forward is the rollout's own forward
   pass (the same launches and bits as ``lbm_step``), and the states it keeps
   go to the backward with the output, so the backward does not re-run the
@@ -71,11 +72,11 @@ def lbm_step(f: torch.Tensor, mask: Optional[torch.Tensor], force: Optional[torc
         # checkpoints, and pass them to the backward on the first output
         from types import SimpleNamespace
 
-        from .adjoint import _forward_impl, segment_length
+        from .adjoint import _forward_impl, default_segment
         rec = SimpleNamespace()
         outs, keep = _forward_impl(rec, f.detach().contiguous(), _rollout_force(force, k.device),
                                    None if tau_field is None else tau_field.detach(), None,
-                                   None, k, int(nsteps), mask, segment_length(int(nsteps), "sqrt"),
+                                   None, k, int(nsteps), mask, default_segment(k, int(nsteps)),
                                    None)
         rec.saved = keep
         outs[0]._lbm_rollout = rec
