@@ -654,8 +654,7 @@ inline bool tri_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa
 template <class T>
 int launch_tri_first(hipStream_t s, const PairArgs& pa) {
   if constexpr (std::is_same<T, f32dev_t>::value) {
-    hipLaunchKernelGGL((k_tri<T, PAIR_LOCAL, PAIR_COLLIDE>), dim3((unsigned)pa.nblocks),
-                       dim3(1024), 0, s, pa);
+    hipLaunchKernelGGL((k_tri<T>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s, pa);
     return check_launch("k_tri launch");
   } else {
     return fail(LBM_ERR_UNSUPPORTED, "three-unit launches are for LBM_F32_DEV");

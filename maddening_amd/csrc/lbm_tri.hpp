@@ -1,30 +1,31 @@
-// Three pull steps per launch for periodic D3Q19 lattices without a force
-// (temporal blocking through LDS, one level deeper than lbm_pair.hpp).
-// Included by lbm_forward.hip.
+// Three units of the pull sequence in one launch for periodic D3Q19
+// lattices without a force (temporal blocking through LDS, one level deeper
+// than lbm_pair.hpp): the collision C of a stored state f_n and two fused
+// steps SC, giving the post-collision state C(f_{n+2}). Included by
+// lbm_forward.hip.
 //
-// The two-step march moves the state across HBM once per two updates and,
+// The two-step march moves the state across HBM once per two units and,
 // in 32-bit storage, is bound by that traffic: with its collisions taken out
-// it runs within 3 % of its time. k_tri runs three units in one pass, so the
-// state crosses HBM once per three updates, and pays for it in arithmetic:
-// 3.67 collisions per output cell against the two-step march's 3.0 at 256^3.
-// In the middle of a sequence that came out even per unit (256^3 f32dev:
-// 253 us against 248-253 for the two-step launch; with the traffic gone the
-// march is bound by issue, two barriers per plane), so lbm_step uses it
-// only where it replaces a single-unit launch: the first launch of an even
-// sequence, collide plus two fused steps (S1 = PAIR_LOCAL), 699 us against
-// 472 + 450 us for a two-step launch and a single one.
+// it runs within 3 % of its time. Three units per pass move it once per
+// three, and pay for it in arithmetic: 3.67 collisions per output cell
+// against the two-step march's 3.0 at 256^3. In the middle of a sequence,
+// as (SC, SC, SC), that came out even per unit (256^3 f32dev: 253 us against
+// 248-253 for the two-step launch; with the traffic gone the march is bound
+// by issue, two barriers per plane), so lbm_step uses k_tri only where it
+// replaces a single-unit launch: the first launch of an even sequence,
+// (C, SC, SC), 699 us against 472 + 450 us for (C, SC) and a single SC.
 //
 // A 1024-thread workgroup owns 15 rows of a c-tile: 64 output columns plus
 // two halo columns each side (kTriW = 68 cells), and marches along axis a
 // over a chunk of planes. Each plane p runs
 //
-//   step 1 at plane p, every cell: gather g_n from HBM (the next plane's
-//          gather is issued first, so it overlaps this plane's arithmetic),
-//          collide;
+//   step 1 at plane p, every cell: load f_n at the cell (the next plane's
+//          loads are issued first, so they overlap this plane's
+//          arithmetic), collide;
 //   step 2 at plane p - 1 on rows 1 .. 13 and columns 1 .. 66: gather step
 //          1's results through exchange buffer x1, collide;
 //   step 3 at plane p - 2 on rows 2 .. 12 and the 64 inner columns: gather
-//          step 2's results through x2, collide, store g_{n+3}.
+//          step 2's results through x2, collide, store.
 //
 // As in the two-step march, each thread keeps its own cell's results of the
 // last two planes of steps 1 and 2 in registers and writes, per plane, the
@@ -39,7 +40,7 @@
 // Threads 0 .. 959 are the inner columns, one row per wave (aligned 64-cell
 // lines); wave 15 holds the 60 halo cells. Each row wave runs with a fixed
 // role (steps 1-3, 1-2 or 1 only), so no store sits in a branch and the
-// wait for the next plane's gather never waits for stores; wave 15 and the
+// wait for the next plane's loads never waits for stores; wave 15 and the
 // waves of a c-tile cut by the lattice's edge decide per lane.
 //
 // Every update is the same inlined moments + collision as k_cell's, and the
@@ -64,7 +65,7 @@ constexpr int kTriOut = kTriRows - 4;   // 11 output rows
 // Roles: TRI_MIXED decides per lane; the others hold for every lane of a wave.
 enum { TRI_MIXED = 0, TRI_S1 = 1, TRI_S2 = 2, TRI_S3 = 3 };
 
-template <class T, int S1, int S3, int ROLE>
+template <class T, int ROLE>
 __device__ __forceinline__ void tri_march(const PairArgs& a, int ct, int bt, int chunk,
                                           float (&x1)[16][1024], float (&x2)[16][1024]) {
   using L = Lattice<19>;
@@ -125,18 +126,10 @@ __device__ __forceinline__ void tri_march(const PairArgs& a, int ct, int bt, int
   };
   auto fetch = [&](int p, C (&v)[19]) {
     const int pc = wrap_a(p);
+    asm volatile("" : "+v"(off[1][1]));
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(off[i][k]));
-    const int pl[3] = {pc == 0 ? na - 1 : pc - 1, pc, pc == na - 1 ? 0 : pc + 1};
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const char* at =
-          S1 == PAIR_LOCAL ? qplane(src, q, pc) + off[1][1]
-                           : qplane(src, q, pl[1 - L::ea[q]]) + off[1 - L::eb[q]][1 - L::ec[q]];
-      v[q] = load_u<L, T, SCALED>(reinterpret_cast<const T*>(at), q);
-    }
+    for (int q = 0; q < 19; ++q)
+      v[q] = load_u<L, T, SCALED>(reinterpret_cast<const T*>(qplane(src, q, pc) + off[1][1]), q);
   };
   auto collide = [&](C (&v)[19]) {
     C drho, rho, ua, ub, uc;
@@ -213,22 +206,7 @@ __device__ __forceinline__ void tri_march(const PairArgs& a, int ct, int bt, int
         g[1] = t1;
         g[2] = t2;
         gather(x2, g);
-        const int cell = ((p - 2) * nb + b) * nc + c;
-        C drho, rho, ua, ub, uc;
-        moments<L, C, false, LS>(g, C(0), C(0), C(0), drho, rho, ua, ub, uc);
-        if constexpr (S3 == PAIR_MOMENTS) {
-          // the stream + moments launch's outputs (lbm.py:970-976)
-          if (a.rho) a.rho[cell] = rho;
-          if (a.p) a.p[cell] = rho * C(1.0 / 3.0);
-          if (a.u) {
-            a.u[3 * cell + 0] = ua;
-            a.u[3 * cell + 1] = ub;
-            a.u[3 * cell + 2] = uc;
-          }
-        } else {
-          collide_for<L, T, C, false, SCALED>(g, a.inv_tau, C(0), C(0), C(0), drho, rho, ua, ub,
-                                              uc);
-        }
+        collide(g);
         const uint32_t po = (uint32_t)((p - 2) * plane * (int)sizeof(T));
 #pragma unroll
         for (int q = 0; q < 19; ++q)
@@ -251,7 +229,7 @@ __device__ __forceinline__ void tri_march(const PairArgs& a, int ct, int bt, int
   for (int p = a0 + 2; p <= a1 + 1; ++p) plane_at(p, std::true_type{});
 }
 
-template <class T, int S1, int S3>
+template <class T>
 __global__ void __launch_bounds__(1024) k_tri(PairArgs a) {
   __shared__ float x1[16][1024];
   __shared__ float x2[16][1024];
@@ -265,14 +243,14 @@ __global__ void __launch_bounds__(1024) k_tri(PairArgs a) {
     const int s2 = wave >= 1 && wave <= kTriRows - 2;
     const int role = __builtin_amdgcn_readfirstlane(s3 ? TRI_S3 : (s2 ? TRI_S2 : TRI_S1));
     if (role == TRI_S3)
-      tri_march<T, S1, S3, TRI_S3>(a, ct, bt, chunk, x1, x2);
+      tri_march<T, TRI_S3>(a, ct, bt, chunk, x1, x2);
     else if (role == TRI_S2)
-      tri_march<T, S1, S3, TRI_S2>(a, ct, bt, chunk, x1, x2);
+      tri_march<T, TRI_S2>(a, ct, bt, chunk, x1, x2);
     else
-      tri_march<T, S1, S3, TRI_S1>(a, ct, bt, chunk, x1, x2);
+      tri_march<T, TRI_S1>(a, ct, bt, chunk, x1, x2);
     return;
   }
-  tri_march<T, S1, S3, TRI_MIXED>(a, ct, bt, chunk, x1, x2);
+  tri_march<T, TRI_MIXED>(a, ct, bt, chunk, x1, x2);
 }
 
 }  // namespace mlbm
