@@ -120,7 +120,9 @@ typedef struct lbm_desc {
  * (a collision, the fused steps, the final stream + moments), keeping the
  * intermediate state in LDS, where the lattice allows it: D3Q19 with
  * LBM_F32 or LBM_F32_DEV, LBM_BB_NONE or LBM_BB_REFERENCE, with or without
- * force and tau fields and pressure faces; D2Q9 periodic with uniform force
+ * force and tau fields and pressure faces, and D3Q19 with LBM_F16_DEV or
+ * LBM_BF16_DEV, periodic without force or fields (under PAIR_FORCE also with
+ * them); D2Q9 periodic with uniform force
  * and tau, its fused steps only (under PAIR_FORCE also LBM_F16_DEV and
  * LBM_BF16_DEV); Q * cells < 2^31. By default only where it measured faster
  * (D3Q19: at least 2^21 cells, dims[0] and dims[1] at least 64, dims[2] one
