@@ -123,10 +123,11 @@ typedef struct lbm_desc {
  * force and tau fields and pressure faces, and D3Q19 with LBM_F16_DEV or
  * LBM_BF16_DEV, periodic without force or fields (under PAIR_FORCE also with
  * them); D2Q9 periodic with uniform force and tau, its fused steps only
- * (under PAIR_FORCE also LBM_F16_DEV and LBM_BF16_DEV); Q * cells < 2^31. By default only where it measured faster
- * (D3Q19: at least 2^21 cells, dims[0] and dims[1] at least 64, dims[2] one
- * of 64, 96, ..., 256 or above 341; D2Q9: at least 2048^2 cells); PAIR_FORCE
- * uses it wherever it applies, PAIR_OFF never. A single D3Q19 update
+ * (under PAIR_FORCE also LBM_F16_DEV and LBM_BF16_DEV); Q * cells < 2^31.
+ * By default only where it measured faster (D3Q19: at least 2^21 cells,
+ * dims[0] and dims[1] at least 64, dims[2] one of 64, 96, ..., 256 or above
+ * 341; D2Q9: at least 2048^2 cells); PAIR_FORCE uses it wherever it
+ * applies, PAIR_OFF never. A single D3Q19 update
  * (nsteps = 1; also 16-bit storage) is one launch unless PAIR_OFF. Results
  * are bit-identical either way. */
 #define LBM_FLAG_PAIR_OFF 16
@@ -135,10 +136,10 @@ typedef struct lbm_desc {
  * D3Q19 sequence run two units per launch has an odd number of units when
  * nsteps is even (4 or more), and lbm_step then runs the collision and the
  * first two fused steps as one three-unit launch instead of a two-unit
- * launch and a single one, where it applies: LBM_F32_DEV or LBM_F16_DEV,
- * LBM_BB_NONE, no force and no fields, dims[2] a multiple of 64, the state
- * within 4 GiB (19 * dims product * 4 or 2 bytes and a plane more).
- * Bit-identical; TRI_OFF turns it off. */
+ * launch and a single one, where it applies: 32- or 16-bit storage (not
+ * LBM_F64), LBM_BB_NONE, no force and no fields, dims[2] a multiple of 64,
+ * the state within 4 GiB (19 * dims product * 4 or 2 bytes and a plane
+ * more). Bit-identical; TRI_OFF turns it off. */
 #define LBM_FLAG_TRI_OFF 64
 
 /* ---- queries ----------------------------------------------------------- */

@@ -596,14 +596,14 @@ inline bool pair_geometry(const lbm_desc* d, const Geom& g, PairArgs* pa) {
   return true;
 }
 
-// The three-step march (k_tri) for float32 and float16 deviation storage,
-// periodic, without force or fields: c-tiles of 64 output columns with two halo
+// The three-step march (k_tri) for 32- and 16-bit storage, periodic,
+// without force or fields: c-tiles of 64 output columns with two halo
 // columns each side (rows of at least 64 cells), the output array within
 // the 4 GiB range of its buffer stores. The chunk length minimises rounds of
 // one-workgroup-per-CU launches x (chunk + 4) planes.
 inline bool tri_geometry(const lbm_desc* d, const Geom& g, PairArgs* pa) {
   if (d->q != 19 || d->bb_mode != LBM_BB_NONE) return false;
-  if (d->storage != LBM_F32_DEV && d->storage != LBM_F16_DEV) return false;
+  if (d->storage == LBM_F64) return false;
   if (d->force[0] != 0.0 || d->force[1] != 0.0 || d->force[2] != 0.0) return false;
   if (g.nc < kTriTile) return false;
   if (19LL * g.plane >= (1LL << 31)) return false;
@@ -642,12 +642,12 @@ inline bool tri_geometry(const lbm_desc* d, const Geom& g, PairArgs* pa) {
 
 // Where lbm_step folds the collision and the first two fused steps into one
 // launch (an even nsteps >= 4 of a sequence run two units per launch): the
-// geometry above with whole 64-column tiles, unless LBM_FLAG_TRI_OFF. 256^3
-// f32dev: 699 us for the three units against 472 + 450 us for (C, SC) and
-// a single SC launch (f16dev: 543-551 against 361 + 230 us; bfloat16 and
-// float32 storage spill registers in the march's loop); in the middle of a
-// sequence (SC, SC, SC) was no faster per unit than (SC, SC) (253 against
-// 248-253 us), so it is used only there.
+// geometry above with whole 64-column tiles, unless LBM_FLAG_TRI_OFF.
+// 256^3, us for the three units against (C, SC) plus a single SC launch:
+// f32dev 654-658 against 480 + 455, f32 755-757 against 502 + 449, f16dev
+// 534-536 against 363 + 234, bf16dev 552-553 against 347 + 233; in the
+// middle of a sequence (SC, SC, SC) was no faster per unit than (SC, SC)
+// (253 against 248-253 us), so it is used only there.
 inline bool tri_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa) {
   if (fields || (d->flags & LBM_FLAG_TRI_OFF) || g.nc % kTriTile != 0) return false;
   return tri_geometry(d, g, pa);
@@ -655,11 +655,11 @@ inline bool tri_plan(const lbm_desc* d, const Geom& g, bool fields, PairArgs* pa
 
 template <class T>
 int launch_tri_first(hipStream_t s, const PairArgs& pa) {
-  if constexpr (std::is_same<T, f32dev_t>::value || std::is_same<T, _Float16>::value) {
+  if constexpr (!std::is_same<T, double>::value) {
     hipLaunchKernelGGL((k_tri<T>), dim3((unsigned)pa.nblocks), dim3(1024), 0, s, pa);
     return check_launch("k_tri launch");
   } else {
-    return fail(LBM_ERR_UNSUPPORTED, "three-unit launches are for LBM_F32_DEV and LBM_F16_DEV");
+    return fail(LBM_ERR_UNSUPPORTED, "three-unit launches are for 32- and 16-bit storage");
   }
 }
 
@@ -1063,9 +1063,7 @@ int step_typed(hipStream_t s, const lbm_desc* d, const Geom& g, const void* f_in
     if constexpr (L::Q == 19) {
       if (nsteps == 1) pair = one_storage<L, T>() && pair_one(d, g, &pa);
       else pair = pair_storage<L, T>() && pair_plan(d, g, ff || tf, &pa);
-      tri = pair && nsteps >= 4 &&
-            (std::is_same<T, f32dev_t>::value || std::is_same<T, _Float16>::value) &&
-            tri_plan(d, g, ff || tf, &ta);
+      tri = pair && nsteps >= 4 && !std::is_same<T, double>::value && tri_plan(d, g, ff || tf, &ta);
     } else {
       pair = pair_storage<L, T>() && PairOf<L>::plan(d, g, ff || tf, &pa);
     }

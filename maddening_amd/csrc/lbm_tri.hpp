@@ -63,7 +63,10 @@ constexpr int kTriRows = 1024 / kTriW;  // 15
 constexpr int kTriOut = kTriRows - 4;   // 11 output rows
 
 // Roles: TRI_MIXED decides per lane; the others hold for every lane of a wave.
-enum { TRI_MIXED = 0, TRI_S1 = 1, TRI_S2 = 2, TRI_S3 = 3 };
+// TRI_H is the wave of halo columns (lanes 960-1023) of a tile the lattice
+// does not cut: step 1 on every lane, step 2 per lane, never step 3 or a
+// store.
+enum { TRI_MIXED = 0, TRI_S1 = 1, TRI_S2 = 2, TRI_S3 = 3, TRI_H = 4 };
 
 template <class T, int ROLE>
 __device__ __forceinline__ void tri_march(const PairArgs& a, int ct, int bt, int chunk,
@@ -95,12 +98,12 @@ __device__ __forceinline__ void tri_march(const PairArgs& a, int ct, int bt, int
   // column before the periodic wrap: -2 .. ctiles * 64 + 1, below 2 * nc
   const int cl = ct * kTriTile - 2 + j;
   const int c = cl < 0 ? cl + nc : (cl >= nc ? cl - nc : cl);
-  const bool active = ROLE != TRI_MIXED || r < kTriRows;
+  constexpr bool PER_LANE = ROLE == TRI_MIXED || ROLE == TRI_H;
+  const bool active = !PER_LANE || r < kTriRows;
   // rows b0 - 2 .. b0 + 12; a thread past them takes the last row's cell
   const int b = (b0 - 2 + (active ? r : kTriRows - 1) + 2 * nb) % nb;
-  const bool in2 = ROLE >= TRI_S2 ||
-                   (ROLE == TRI_MIXED && active && r >= 1 && r <= kTriRows - 2 && j >= 1 &&
-                    j <= W - 2);
+  const bool in2 = ROLE == TRI_S2 || ROLE == TRI_S3 ||
+                   (PER_LANE && active && r >= 1 && r <= kTriRows - 2 && j >= 1 && j <= W - 2);
   const bool in3 = ROLE == TRI_S3 ||
                    (ROLE == TRI_MIXED && r >= 2 && r <= kTriRows - 3 && b0 + r - 2 < nb &&
                     j >= 2 && j < W - 2 && cl < nc);
@@ -200,7 +203,7 @@ __device__ __forceinline__ void tri_march(const PairArgs& a, int ct, int bt, int
     }
     __syncthreads();
     if constexpr (WITH3) {
-      if (ROLE != TRI_S1 && ROLE != TRI_S2 && in3) {
+      if (ROLE != TRI_S1 && ROLE != TRI_S2 && ROLE != TRI_H && in3) {
         C g[19];
         g[0] = t0;
         g[1] = t1;
@@ -248,6 +251,10 @@ __global__ void __launch_bounds__(1024) k_tri(PairArgs a) {
       tri_march<T, TRI_S2>(a, ct, bt, chunk, x1, x2);
     else
       tri_march<T, TRI_S1>(a, ct, bt, chunk, x1, x2);
+    return;
+  }
+  if (wave == kTriRows && (ct + 1) * kTriTile <= a.nc) {
+    tri_march<T, TRI_H>(a, ct, bt, chunk, x1, x2);
     return;
   }
   tri_march<T, TRI_MIXED>(a, ct, bt, chunk, x1, x2);

@@ -60,8 +60,7 @@ class LBMKernel:
         (LBM_FLAG_PAIR_*); "auto" uses them where they measured faster.
         Results are bit-identical in every mode.
     tri : False sets LBM_FLAG_TRI_OFF (no three-unit first launch in even
-        sequences of float32 or float16 deviation storage); the same bits
-        either way.
+        sequences of 32- and 16-bit storage); the same bits either way.
     """
 
     def __init__(self, grid_shape: Sequence[int], lattice: str = "D3Q19",

@@ -108,8 +108,8 @@ def test_16_bit_auto_pairs_equal_single_launches(storage):
     for pair in ("auto", "off"):
         k = LBMKernel(shape, lattice="D3Q19", tau=0.8, storage=storage, pair=pair)
         if pair == "auto":
-            # float16: (C, SC, SC) first, then 4 pairs and the last
-            assert k.schedule(10) == ((5, 5) if storage == "f16dev" else (6, 5))
+            # (C, SC, SC) first, then 3 pairs and the last
+            assert k.schedule(10) == (5, 5)
         f = k.equilibrium(torch.from_numpy(rho).float(), torch.from_numpy(u).float())
         out[pair] = [t.cpu() for t in k.step(f, 10)]
         torch.cuda.synchronize()
@@ -311,7 +311,7 @@ def _phase(k, phase, src, dst):
 
 
 @pytest.mark.parametrize("shape", TRI_SHAPES)
-@pytest.mark.parametrize("storage", ["f32dev", "f16dev"])
+@pytest.mark.parametrize("storage", ["f32", "f32dev", "f16dev", "bf16dev"])
 def test_three_unit_launch_equals_three_single_launches_bitwise(shape, storage):
     """Phase 7, (C, SC, SC) in one launch, against phase 0 then phase 1 twice."""
     from maddening_amd.lbm import LBMKernel
@@ -331,7 +331,7 @@ def test_three_unit_launch_equals_three_single_launches_bitwise(shape, storage):
 @pytest.mark.parametrize("shape,pair", [((3, 5, 64), "force"), ((4, 23, 192), "force"),
                                         ((64, 128, 256), "auto")])
 @pytest.mark.parametrize("nsteps", [4, 6, 20])
-@pytest.mark.parametrize("storage", ["f32dev", "f16dev"])
+@pytest.mark.parametrize("storage", ["f32", "f32dev", "f16dev", "bf16dev"])
 def test_even_sequences_with_a_three_unit_launch_equal_single_launches(shape, pair, nsteps,
                                                                         storage):
     print(f"lbm_step with (C, SC, SC): {shape} {pair} {storage} nsteps={nsteps}", flush=True)

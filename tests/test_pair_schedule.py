@@ -42,7 +42,7 @@ def test_auto_pairs_at_256_cubed(storage, nsteps):
     from maddening_amd import _lib
     d = _desc((256, 256, 256), storage, flags=_lib.LBM_FLAG_TRI_OFF)
     assert _schedule(d, nsteps) == _folded(nsteps)
-    if storage == 0 or nsteps % 2 or nsteps < 4:
+    if nsteps % 2 or nsteps < 4:
         assert _schedule(_desc((256, 256, 256), storage), nsteps) == _folded(nsteps)
 
 
@@ -53,20 +53,20 @@ def _tri(nsteps):
 
 @pytest.mark.parametrize("nsteps", [4, 6, 20, 1000])
 def test_even_f32dev_sequences_start_with_three_units(nsteps):
-    """An even nsteps >= 4 has an odd number of units; with float32 or
-    float16 deviation storage, periodic, no force, rows of whole 64-column
-    tiles, the first launch runs three units instead of (C, SC) plus a
-    single SC."""
+    """An even nsteps >= 4 has an odd number of units; with 32- or 16-bit
+    storage, periodic, no force, rows of whole 64-column tiles, the first
+    launch runs three units instead of (C, SC) plus a single SC."""
     from maddening_amd import _lib
     for dims in [(256, 256, 256), (128, 128, 512), (64, 256, 128), (512, 256, 128)]:
         assert _schedule(_desc(dims), nsteps) == _tri(nsteps), dims
-    for dims in [(256, 256, 256), (64, 256, 128)]:
-        assert _schedule(_desc(dims, storage=2), nsteps) == _tri(nsteps), dims
+    for storage in (0, 2, 3):  # f32, f16dev, bf16dev
+        for dims in [(256, 256, 256), (64, 256, 128)]:
+            assert _schedule(_desc(dims, storage=storage), nsteps) == _tri(nsteps), dims
     # forced two-step launches on a small lattice take it too
     assert _schedule(_desc((3, 5, 64), flags=_lib.LBM_FLAG_PAIR_FORCE), nsteps) == _tri(nsteps)
-    # not: float32, bfloat16, a force, fields, the reference wall rule, rows
-    # that are not whole tiles, PAIR_OFF, TRI_OFF
-    for d in [_desc((256, 256, 256), storage=0), _desc((256, 256, 256), storage=3),
+    # not: a force, fields, the reference wall rule, rows that are not whole
+    # tiles, PAIR_OFF, TRI_OFF
+    for d in [_desc((256, 256, 256), storage=0, flags=_lib.LBM_FLAG_TRI_OFF),
               _desc((256, 256, 256), storage=2, flags=_lib.LBM_FLAG_TRI_OFF),
               _desc((256, 256, 256), force=(0.0, 1e-5, 0.0)),
               _desc((256, 256, 256), bb=1), _desc((160, 160, 160)),
@@ -165,9 +165,9 @@ def test_phase_3_rejected_where_pairs_do_not_apply():
     assert L.lbm_pull_phase(None, ctypes.byref(dh), 6, ctypes.c_void_p(4096),
                             ctypes.c_void_p(8192), ctypes.c_void_p(64), None, None, None, None,
                             None, None) == -2
-    # phase 7 (C, SC, SC): float32 or float16 deviation storage, periodic, no
-    # force, no fields, rows of at least 64 cells
-    for dd in [d, _desc((8, 8, 64), storage=0), _desc((8, 8, 64), storage=3),
+    # phase 7 (C, SC, SC): 32- or 16-bit storage, periodic, no force, no
+    # fields, rows of at least 64 cells
+    for dd in [d, _desc((8, 8, 64), storage=1),
                _desc((8, 8, 32)), _desc((8, 8, 64), force=(1e-5, 0.0, 0.0))]:
         assert L.lbm_pull_phase(None, ctypes.byref(dd), 7, ctypes.c_void_p(4096),
                                 ctypes.c_void_p(8192), None, None, None, None, None, None,
