@@ -157,6 +157,47 @@ def test_spec_derivatives_fluxes():
 
 
 @gpu
+@pytest.mark.parametrize("face", ["x_min", "x_max", "y_max", "z_min"])
+def test_outlet_pressure_average_after_updates(face):
+    """compute_boundary_fluxes (lbm.py:995-1017) after eight updates with
+    walls and a force: the mean of the float64 oracle's pressure over the
+    fluid cells of the outlet face (wall cells left out of sum and count);
+    a face that is all wall gives 0 (the count is clamped to 1)."""
+    import torch
+    from oracle import lbm_numpy as orc
+    rng = np.random.default_rng(17)
+    shape = (7, 9, 10)
+    wall = rng.random(shape) < 0.3
+    F = np.zeros(shape + (3,))
+    F[..., 0] = 2e-4
+    F[..., 1] = -1e-4
+    node = LBMNode(name="lbm", timestep=0.001, grid_shape=shape, wall_mask=wall, viscosity=0.08,
+                   outlet_face=face)
+    st = node.initial_state()
+    for _ in range(8):
+        st = node.update(st, {"body_force": torch.as_tensor(F, dtype=torch.float32)}, 0.001)
+    got = float(node.compute_boundary_fluxes(st, {}, node.delta_t)["outlet_pressure_avg"])
+    ref = orc.initial_state(shape, 19, mask=wall.astype(np.uint8))
+    out = orc.run(ref["f"], 8, tau=node.tau, force=F.astype(np.float32).astype(np.float64),
+                  mask=wall.astype(np.uint8))
+    axis, side = {"x_min": (0, 0), "x_max": (0, -1), "y_max": (1, -1), "z_min": (2, 0)}[face]
+    p_face = np.take(out["pressure"], side, axis=axis)
+    fluid = ~np.take(wall, side, axis=axis)
+    want = p_face[fluid].sum() / max(int(fluid.sum()), 1)
+    assert fluid.sum() > 0
+    assert got == pytest.approx(want, rel=1e-6)
+    # the same face with every cell a wall
+    wall2 = wall.copy()
+    idx = [slice(None)] * 3
+    idx[axis] = side
+    wall2[tuple(idx)] = True
+    node2 = LBMNode(name="lbm", timestep=0.001, grid_shape=shape, wall_mask=wall2, viscosity=0.08,
+                    outlet_face=face)
+    st2 = node2.update(node2.initial_state(), {}, 0.001)
+    assert float(node2.compute_boundary_fluxes(st2, {}, node2.delta_t)["outlet_pressure_avg"]) == 0.0
+
+
+@gpu
 def test_node_matches_oracle_with_walls_and_force():
     """Ten node updates vs the float64 oracle (f32 storage: 1e-5 relative)."""
     import torch
