@@ -519,19 +519,10 @@ __global__ void __launch_bounds__(1024) k_pair(PairArgs a) {
   const int rest = lb / a.ctiles;
   const int ct = lb - rest * a.ctiles, bt = rest % a.btiles, chunk = rest / a.btiles;
   if constexpr (FORM <= PAIR_GEN && S2 == PAIR_COLLIDE) {
-    // whole rows in whole waves, or rows cut into whole 64-column tiles
-    // (wave r < R holds row r's inner columns, the last wave the halo
-    // columns, step 1 only): one role per wave (the barrier count per plane
-    // is the same in both roles). 128x128x1024, two runs on two boxes, us
-    // per launch against the per-lane form: f32dev 575-585 against 578-581
-    // (even), f32 587-593 against 596-600, f16dev 384-388 against 391-395;
-    // rows cut with a partial last tile keep the per-lane form
-    // (128x128x1000: 725 with roles in the whole tiles against 699 us)
-    const bool whole = !a.halo && a.w % 64 == 0;
-    const bool tile = a.halo && a.tc == 64 && a.nc % 64 == 0;
-    if (whole || tile) {
-      const int tid = (int)threadIdx.x;
-      const int r = whole ? tid / a.w : (tid < (a.tb + 2) * 64 ? tid / 64 : -1);
+    // whole rows in whole waves: one role per wave (the barrier count per
+    // plane is the same in both roles)
+    if (!a.halo && a.w % 64 == 0) {
+      const int r = (int)threadIdx.x / a.w;
       const bool in = r >= 1 && r <= a.tb && bt * a.tb + r - 1 < a.nb;
       if (__builtin_amdgcn_readfirstlane(in ? 1 : 0))
         pair_march<T, S1, S2, FORM, PAIR_INNER, BUF>(a, ct, bt, chunk, x);
