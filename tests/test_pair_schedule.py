@@ -52,7 +52,7 @@ def _tri(nsteps):
 
 
 @pytest.mark.parametrize("nsteps", [4, 6, 20, 1000])
-def test_even_f32dev_sequences_start_with_three_units(nsteps):
+def test_even_sequences_start_with_three_units(nsteps):
     """An even nsteps >= 4 has an odd number of units; with 32- or 16-bit
     storage, periodic, no force, rows of whole 64-column tiles, the first
     launch runs three units instead of (C, SC) plus a single SC."""
