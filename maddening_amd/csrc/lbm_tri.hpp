@@ -13,7 +13,8 @@
 // 248-253 for the two-step launch; with the traffic gone the march is bound
 // by issue, two barriers per plane), so lbm_step uses k_tri only where it
 // replaces a single-unit launch: the first launch of an even sequence,
-// (C, SC, SC), 699 us against 472 + 450 us for (C, SC) and a single SC.
+// (C, SC, SC), 654-658 us against 480 + 455 us for (C, SC) and a single SC
+// (f32dev; every 32- and 16-bit storage gains, the figures at tri_plan).
 //
 // A 1024-thread workgroup owns 15 rows of a c-tile: 64 output columns plus
 // two halo columns each side (kTriW = 68 cells), and marches along axis a
@@ -40,8 +41,11 @@
 // Threads 0 .. 959 are the inner columns, one row per wave (aligned 64-cell
 // lines); wave 15 holds the 60 halo cells. Each row wave runs with a fixed
 // role (steps 1-3, 1-2 or 1 only), so no store sits in a branch and the
-// wait for the next plane's loads never waits for stores; wave 15 and the
-// waves of a c-tile cut by the lattice's edge decide per lane.
+// wait for the next plane's loads never waits for stores. Wave 15 has a role
+// of its own (step 1, step 2 per lane, no step 3 and no store), which took
+// the f32dev launch from 695-699 to 654-658 us and keeps float32 and
+// bfloat16 storage from spilling in the loop; the waves of a c-tile cut by
+// the lattice's edge decide per lane.
 //
 // Every update is the same inlined moments + collision as k_cell's, and the
 // results of steps 1 and 2 pass through the storage type's encode/decode as
