@@ -43,9 +43,10 @@
 // role (steps 1-3, 1-2 or 1 only), so no store sits in a branch and the
 // wait for the next plane's loads never waits for stores. Wave 15 has a role
 // of its own (step 1, step 2 per lane, no step 3 and no store), which took
-// the f32dev launch from 695-699 to 654-658 us and keeps float32 and
-// bfloat16 storage from spilling in the loop; the waves of a c-tile cut by
-// the lattice's edge decide per lane.
+// the f32dev launch from 695-699 to 654-658 us and keeps float32 storage
+// from spilling in the loop (bfloat16's halo wave still spills a few
+// registers there); the waves of a c-tile cut by the lattice's edge decide
+// per lane.
 //
 // Every update is the same inlined moments + collision as k_cell's, and the
 // results of steps 1 and 2 pass through the storage type's encode/decode as
