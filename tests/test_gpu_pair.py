@@ -328,6 +328,36 @@ def test_three_unit_launch_equals_three_single_launches_bitwise(shape, storage):
     assert torch.equal(a, t)
 
 
+def test_three_unit_launch_random_shapes_bitwise():
+    """Seeded random extents (rows of 64-1100 cells, whole 64-column tiles
+    or a partial last one; other axes 1-40), random tau, every 32- and
+    16-bit storage: phase 7 equals phase 0 then phase 1 twice bit for bit."""
+    from maddening_amd.lbm import LBMKernel
+    rng = np.random.default_rng(77)
+    storages = ("f32", "f32dev", "f16dev", "bf16dev")
+    for case in range(24):
+        nc = 64 * int(rng.integers(1, 18)) if case % 2 == 0 else int(rng.integers(64, 1101))
+        na = int(rng.integers(1, 41))
+        nb = int(rng.integers(1, 41))
+        while na * nb * nc > 150_000:
+            na = max(1, na // 2)
+            nb = max(1, nb // 2)
+        shape = (na, nb, nc)
+        storage = storages[case % 4]
+        tau = float(rng.uniform(0.52, 1.8))
+        print(f"three units, random case {case}: {shape} {storage} tau={tau:.3f}", flush=True)
+        k = LBMKernel(shape, lattice="D3Q19", tau=tau, storage=storage, scheme="pull")
+        f0 = _state(k, shape, case)
+        a, b = k.empty(), k.empty()
+        _phase(k, 0, f0, a)
+        _phase(k, 1, a, b)
+        _phase(k, 1, b, a)
+        t = k.empty()
+        _phase(k, 7, f0, t)
+        torch.cuda.synchronize()
+        assert torch.equal(a, t), (case, shape, storage)
+
+
 @pytest.mark.parametrize("shape,pair", [((3, 5, 64), "force"), ((4, 23, 192), "force"),
                                         ((64, 128, 256), "auto")])
 @pytest.mark.parametrize("nsteps", [4, 6, 20])
