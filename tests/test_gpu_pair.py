@@ -328,6 +328,39 @@ def test_three_unit_launch_equals_three_single_launches_bitwise(shape, storage):
     assert torch.equal(a, t)
 
 
+def test_16_bit_pair_launches_random_shapes_bitwise():
+    """Seeded random extents (rows of 1-1200 cells, whole or in 64-column
+    tiles; other axes 1-40), step counts 2-9, random tau, float16 and
+    bfloat16 storage without a force (the plain form the library picks by
+    itself) and with one (the forced general form): two-unit launches equal
+    one launch per step bit for bit."""
+    from maddening_amd.lbm import LBMKernel
+    rng = np.random.default_rng(4048)
+    for case in range(16):
+        nc = int(rng.integers(1, 342)) if case < 12 else int(rng.integers(342, 1201))
+        na = int(rng.integers(1, 41))
+        nb = int(rng.integers(1, 41))
+        while na * nb * nc > 150_000:
+            na = max(1, na // 2)
+            nb = max(1, nb // 2)
+        shape = (na, nb, nc)
+        nsteps = int(rng.integers(2, 10))
+        storage = ("f16dev", "bf16dev")[case % 2]
+        tau = float(rng.uniform(0.52, 1.8))
+        force = [0.0, 0.0, 0.0] if case % 4 < 2 else [float(v) for v in rng.uniform(-3e-5, 3e-5, 3)]
+        print(f"16-bit random case {case}: {shape} {storage} nsteps={nsteps} tau={tau:.3f} "
+              f"force={force}", flush=True)
+        outs = []
+        for pair in ("force", "off"):
+            k = LBMKernel(shape, lattice="D3Q19", tau=tau, storage=storage, scheme="pull",
+                          pair=pair)
+            f = _state(k, shape, case)
+            outs.append([t.cpu() for t in k.step(f, nsteps, force=force)])
+        torch.cuda.synchronize()
+        for a, b in zip(*outs):
+            assert torch.equal(a, b), (case, shape, storage, nsteps)
+
+
 def test_three_unit_launch_random_shapes_bitwise():
     """Seeded random extents (rows of 64-1100 cells, whole 64-column tiles
     or a partial last one; other axes 1-40), random tau, every 32- and
