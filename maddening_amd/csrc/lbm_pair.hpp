@@ -547,11 +547,11 @@ inline int pair_form(const PairArgs& a) {
 // on either side; a 1024-thread workgroup marches along b over a chunk of
 // rows. Step 1 runs on all 1024 columns (the halos wrap periodically), step
 // 2 on the 960 inner ones, reading its left and right neighbours' step-1
-// values from LDS (24 KB: the six directions with e_c != 0). Same structure
-// and same bits as the D3Q19 march above, with rows in place of planes. It
-// takes 68 VGPRs (7 waves per SIMD: one 16-wave workgroup per CU); capped at
-// 64 so that two fit, it spills 3 and is no faster (8192^2: 133.4k against
-// 133.5k MLUPS, 2048^2: 81.7k against 85.0k). With 16-bit storage each lane
+// values from LDS (two halves of 24 KB: the six directions with e_c != 0).
+// Same structure and same bits as the D3Q19 march above, with rows in place
+// of planes. It takes 68 VGPRs (7 waves per SIMD: one 16-wave workgroup per
+// CU); capped at 64 so that two fit, it spilled 3 and was no faster (8192^2:
+// 133.4k against 133.5k MLUPS, 2048^2: 81.7k against 85.0k). With 16-bit storage each lane
 // moves 2-byte values, and it is 2.2x slower than the row kernels that move
 // 2-4 cells per lane (8192^2 f16dev: 71.6k against 158.8k MLUPS), so it runs
 // there only under LBM_FLAG_PAIR_FORCE.
@@ -573,7 +573,7 @@ __global__ void __launch_bounds__(1024) k_pair9(Pair9Args a) {
   using St = Storage<T>;
   using C = float;
   static_assert(sizeof(typename Storage<T>::C) == 4, "float compute only");
-  __shared__ float x[6][1024];  // q = 3 .. 8
+  __shared__ float xx[2][6][1024];  // q = 3 .. 8, rows alternating between halves
   const int ct = (int)blockIdx.x % a.ctiles;
   const int b0 = ((int)blockIdx.x / a.ctiles) * a.lb;
   const int b1 = b0 + a.lb < a.nb ? b0 + a.lb : a.nb;  // output rows [b0, b1)
@@ -608,6 +608,14 @@ __global__ void __launch_bounds__(1024) k_pair9(Pair9Args a) {
       as_stored<L, T, C>(d);  // the values a store and reload would give
     }
     const bool out = p > b0;
+    // rows alternate between the two halves, so one barrier per row both
+    // publishes this row's values and retires the reads of the half
+    // written next (8192^2 f32dev 956 against 969 us per launch with one
+    // half and two barriers, f32 937 against 957, 4096^2 254 against 264,
+    // 2048^2 83 against 88; issuing the next row's gather before this row's
+    // arithmetic as well was faster at 2048^2 and 4096^2 but slower at
+    // 8192^2, 1017 against 969)
+    float (&x)[6][1024] = xx[p & 1];
     if (out) {
 #pragma unroll
       for (int q = 3; q < 9; ++q)
@@ -629,7 +637,6 @@ __global__ void __launch_bounds__(1024) k_pair9(Pair9Args a) {
 #pragma unroll
       for (int q = 0; q < 9; ++q) St::template store<L>(dst + q * P + cell, q, e[q]);
     }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
       if (L::eb[q] > 0) {
