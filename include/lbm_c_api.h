@@ -126,7 +126,7 @@ typedef struct lbm_desc {
  * (under PAIR_FORCE also LBM_F16_DEV and LBM_BF16_DEV); Q * cells < 2^31.
  * By default only where it measured faster (D3Q19: at least 2^21 cells,
  * dims[0] and dims[1] at least 64, dims[2] one of 64, 96, ..., 256 or above
- * 341; D2Q9: at least 2048^2 cells); PAIR_FORCE uses it wherever it
+ * 341; D2Q9: at least 2^21 cells); PAIR_FORCE uses it wherever it
  * applies, PAIR_OFF never. A single D3Q19 update
  * (nsteps = 1; also 16-bit storage) is one launch unless PAIR_OFF. Results
  * are bit-identical either way. */

@@ -741,10 +741,11 @@ inline bool pair_one(const lbm_desc* d, const Geom& g, PairArgs* pa) {
 // under LBM_FLAG_PAIR_FORCE: 8192^2 f16dev 71.6k MLUPS against 158.8k for
 // the row kernel), uniform force and tau, no pressure faces, 9 planes
 // addressable with 32-bit offsets. Without LBM_FLAG_PAIR_FORCE only 32-bit
-// storage and only from 2048^2 cells on, where it measured faster
-// (f32dev, per two steps against two k_cell launches: 8192^2 1006 against
-// 1716 us, 4096^2 339 against 416, 2048^2 104 against 140, 2048 x 8192 350
-// against 419; 1024^2 and 512^2 lost at every chunk length).
+// storage and only from 2^21 cells on, where it measured faster (f32dev,
+// per two steps against two one-step launches, with the march's two LDS
+// halves: 8192^2 956 against 1750 us, 2048^2 82.5 against 107.8, 1536^2
+// 45.1 against 68.1; 1024^2 35.6 against 39.5 but f32 even, 39.4 against
+// 39.6; 512^2 24 against 14).
 inline bool pair9_plan(const lbm_desc* d, const Geom& g, bool fields, Pair9Args* pa) {
   if (d->flags & LBM_FLAG_PAIR_OFF) return false;
   const bool force = (d->flags & LBM_FLAG_PAIR_FORCE) != 0;
@@ -753,7 +754,7 @@ inline bool pair9_plan(const lbm_desc* d, const Geom& g, bool fields, Pair9Args*
   if (!force && d->storage != LBM_F32 && d->storage != LBM_F32_DEV) return false;
   if (d->inlet_face > 0 || d->outlet_face > 0) return false;
   if (9LL * g.plane >= (1LL << 31)) return false;
-  if (!force && g.plane < 2048LL * 2048) return false;
+  if (!force && g.plane < (1LL << 21)) return false;
   // chunk rows by the same rounds model as the D3Q19 plan (measured best:
   // 8192^2 and 2048^2 32 rows, 4096^2 16 rows: 311 against 339 us)
   const long long ctiles = (g.nc + 959) / 960;

@@ -280,9 +280,12 @@ def test_d2q9_pair_launches_match_oracle_f32():
     assert np.max(np.abs(u.cpu().numpy() - ref["velocity"])) / np.max(np.abs(ref["velocity"])) < 1e-5
 
 
-def test_d2q9_auto_pairs_at_2048_squared_equal_single_launches():
+@pytest.mark.parametrize("shape", [(2048, 2048), (1536, 1536), (1024, 2048)])
+def test_d2q9_auto_pairs_equal_single_launches(shape):
+    """Periodic D2Q9 lattices of at least 2^21 cells take two-step launches
+    by themselves (a partial last 960-column tile at 1536 and 2048 columns);
+    9 steps equal one launch per step bit for bit."""
     from maddening_amd.lbm import LBMKernel
-    shape = (2048, 2048)
     outs = []
     for pair in ("auto", "off"):
         k = LBMKernel(shape, lattice="D2Q9", tau=0.9, storage="f32dev", pair=pair)

@@ -177,9 +177,9 @@ def test_phase_3_rejected_where_pairs_do_not_apply():
                             ctypes.c_void_p(8192), None, None, None, None, None, None, None) == -1
 
 
-def test_d2q9_pairs_from_2048_squared():
+def test_d2q9_pairs_from_2_to_the_21_cells():
     from maddening_amd import _lib
-    for dims in [(2048, 2048), (8192, 8192), (2048, 8192)]:
+    for dims in [(2048, 2048), (8192, 8192), (2048, 8192), (1536, 1536), (1024, 2048)]:
         assert _schedule(_desc(dims, q=9), 200) == (2 + 99 + 1, 99)
     for d in [_desc((1024, 1024), q=9), _desc((8192, 8192), q=9, storage=2),
               _desc((8192, 8192), q=9, bb=2), _desc((8192, 8192), q=9, flags=_lib.LBM_FLAG_PAIR_OFF)]:
